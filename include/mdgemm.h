@@ -1,0 +1,197 @@
+/*
+ * mdgemm.h -- C ABI of the MI355X (gfx950) multiple-double GEMM library
+ * (libmdgemm.so).
+ *
+ * Numbers are p-doubles: L in {2, 4, 8, 16} FP64 limbs (double, quad, octo
+ * and hexa-double), most significant limb first.  Matrices use the
+ * reference's interleaved "component-array" layout: entry (r, c) of a
+ * row-major matrix with leading dimension ld (in entries) occupies
+ * X[(r * ld + c) * L + limb]  (C++/TCcuda.cu:130-131,154-155).
+ *
+ * Every matrix pointer is a DEVICE pointer (hipMalloc'd by the caller); the
+ * library allocates only its own scratch.  `stream` is a hipStream_t (NULL =
+ * the null stream); every call is asynchronous on it unless noted.
+ * Every entry point returns an mdg_status; mdg_last_error() gives the text
+ * of the last failure on the calling thread.
+ *
+ * The reference has no error reporting at all (an unsupported p silently
+ * returns zeros, C++/TCcuda.cu:269-297); here unsupported limb counts, bad
+ * shapes, null pointers and non-finite inputs are rejected.
+ */
+#ifndef MDGEMM_H
+#define MDGEMM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mdg_status {
+    MDG_SUCCESS = 0,
+    MDG_ERROR_INVALID_VALUE = 1,   /* bad L, shape, leading dimension, pointer */
+    MDG_ERROR_NOT_SUPPORTED = 2,   /* valid request this build cannot serve */
+    MDG_ERROR_HIP = 3,             /* a HIP runtime call failed */
+    MDG_ERROR_OUT_OF_MEMORY = 4,   /* scratch allocation failed */
+    MDG_ERROR_NON_FINITE = 5,      /* an input limb is Inf or NaN */
+    MDG_ERROR_RANGE = 6            /* exponent range beyond the slice scheme */
+} mdg_status;
+
+/* GEMM methods for mdg_gemm / mdg_gemm_batched */
+typedef enum mdg_method {
+    MDG_METHOD_VECTOR = 0,      /* one thread per output entry, p-double mul+add */
+    MDG_METHOD_F64_SPLIT = 1,   /* signed fixed-point slices on FP64 MFMA */
+    MDG_METHOD_INT8_OZAKI = 2,  /* 8-bit slices on INT8 MFMA, INT32 accumulate */
+    MDG_METHOD_BF16_OZAKI = 3   /* 8-bit slices on BF16 MFMA, FP32 accumulate */
+} mdg_method;
+
+/* two_prod flavour of the vector path */
+typedef enum mdg_twoprod {
+    MDG_TWOPROD_FMA = 0,     /* p = a*b, e = fma(a,b,-p) */
+    MDG_TWOPROD_DEKKER = 1   /* the reference's Veltkamp/Dekker product */
+} mdg_twoprod;
+
+/* What a split-product GEMM did (filled when a non-NULL pointer is given). */
+typedef struct mdg_gemm_info {
+    int method;            /* mdg_method actually run */
+    int slices;            /* S: slices per operand entry */
+    int slice_bits;        /* w: bits per slice */
+    int diagonals;         /* slice-sum diagonals kept (== slices) */
+    long long pair_gemms;  /* P: slice-pair products issued, S(S+1)/2 */
+    int total_bits;        /* W = S * w fixed-point bits per entry */
+    int spread_a;          /* exponent spread estimate of A, bits */
+    int spread_b;          /* exponent spread estimate of B, bits */
+    int flushed;           /* 1 if accumulators were widened inside the K loop */
+} mdg_gemm_info;
+
+/* One problem of a batch (all pointers on the device). */
+typedef struct mdg_problem {
+    int L, m, n, k;
+    const double *A;
+    long long lda;
+    const double *B;
+    long long ldb;
+    double *C;
+    long long ldc;
+} mdg_problem;
+
+const char *mdg_last_error(void);
+int mdg_version(void);
+
+/* Initialise the library on the current device (optional; every entry point
+   initialises on first use). */
+int mdg_init(void);
+
+/* Release cached scratch buffers of the current device. */
+int mdg_release_workspace(void);
+
+/* ----------------------------------------------------------------------
+ * Vector GEMM.  C = A * B with one thread per output entry, each a k-step
+ * chain acc = add(acc, mul(A[r][k], B[k][c])), acc starting at zero, k
+ * ascending -- the reference kernels ddmm/qdmm/odmm/hdmm
+ * (C++/TCcuda.cu:119-246) and its host loops matmul{dd,qd,od,hd}f
+ * (C++/TCcuda.cu:320-372), generalised to m x k times k x n with leading
+ * dimensions.  Bit-identical to the reference's host arithmetic with
+ * MDG_TWOPROD_DEKKER, and with MDG_TWOPROD_FMA whenever no partial product
+ * underflows.
+ * -------------------------------------------------------------------- */
+int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda,
+                    const double *B, long long ldb, double *C, long long ldc,
+                    int twoprod, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Split-product GEMM C = A * B (no reference counterpart as a single call;
+ * it replaces the reference's split -> tensor-core GEMM -> squeeze chain,
+ * C++/TCtest.cu:53-114, for signed inputs of any exponent range).
+ *   method       MDG_METHOD_F64_SPLIT / _INT8_OZAKI / _BF16_OZAKI, or
+ *                MDG_METHOD_VECTOR (then identical to mdg_vector_gemm, FMA)
+ *   target_bits  fixed-point precision goal; <= 0 means 53*L.  The slice
+ *                count is chosen from it and from the exponent spread of
+ *                each operand's rows/columns (a small device->host read).
+ * Synchronises `stream` once (to read the exponent spread).
+ * -------------------------------------------------------------------- */
+int mdg_gemm(int L, int m, int n, int k, const double *A, long long lda,
+             const double *B, long long ldb, double *C, long long ldc,
+             int method, int target_bits, mdg_gemm_info *info, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Batched GEMM: `count` independent problems (L in {2,4,8,16} may differ
+ * per problem), all computed with `method`.  `problems` is a HOST array.
+ *   spread_bits  exponent-spread allowance per operand (bits) used to size
+ *                the slices without reading the data back; < 0 means 16.
+ *   persistent   1: one persistent launch per stage over a device work
+ *                list, captured once into a hipGraph and replayed on later
+ *                calls with the same batch shape; 0: a loop of per-problem
+ *                launches (the baseline).
+ * -------------------------------------------------------------------- */
+int mdg_gemm_batched(int count, const mdg_problem *problems, int method,
+                     int target_bits, int spread_bits, int persistent,
+                     void *stream);
+
+/* ----------------------------------------------------------------------
+ * The reference's split ("tensor core") path, piece by piece.
+ * -------------------------------------------------------------------- */
+
+/* split4pd (C++/floatybits.cu:196-206, balance4 :150-154): each limb of
+   `count` doubles (count a multiple of p) becomes 4 parts of 13/13/13/13
+   stored fraction bits; adjacent parts of one p-double are exponent-balanced.
+   out holds 4*count doubles.  The reference's bit semantics are kept,
+   including that a negative limb splits into zero parts (floatybits.cu:20-28)
+   and that the 53rd significand bit is dropped (:97-99). */
+int mdg_split4pd(const double *x, long long count, int p, double *out, void *stream);
+
+/* split8pd (C++/floatybits.cu:303-313, balance8 :208-216): 8 parts of
+   7,7,7,7,6,6,12 bits plus remainder; out holds 8*count doubles. */
+int mdg_split8pd(const double *x, long long count, int p, double *out, void *stream);
+
+/* mixsplit2 (C++/floatybits.cu:327-342, mixbalance :315-325): double-doubles,
+   high limb in 4 parts and low limb in 8; out holds 6*count doubles. */
+int mdg_mixsplit2(const double *x, long long count, double *out, void *stream);
+
+/* bigB2 (C++/TCcuda.cu:79-95): the (n*pp) x (n*pp) column-major block
+   Toeplitz expansion of an n x n matrix of pp-part entries.  Provided for
+   callers that want the explicit operand; mdg_conv_gemm never builds it. */
+int mdg_bigB2(const double *Bq, int n, int pp, double *out, void *stream);
+
+/* The reference's FP64 tensor-core kernel `matmul` (C++/TCtest.cu:21-46):
+   C (M x N, row-major) += A (M x K, row-major) * B (K x N, column-major), on
+   FP64 MFMA.  Accumulates into C like the reference. */
+int mdg_dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C,
+                  void *stream);
+
+/* Limb convolution C1q = Aq * bigB2(Bq) without materialising bigB2:
+   C1q[r][c][i] = sum_k sum_{j<=i} Aq[r][k][j] * Bq[k][c][i-j] for an
+   m x k and a k x n matrix of pp-part entries (C++/TCcuda.cu:79-95 with
+   C++/TCtest.cu:21-46, square n in the reference).  FP64 MFMA. */
+int mdg_conv_gemm(int m, int n, int k, int pp, const double *Aq, const double *Bq,
+                  double *C1q, void *stream);
+
+/* pllsqueeze_old (C++/TCcuda.cu:876-958): y[i] = fold_j inc_d(y[i], x[i*pp+j])
+   from zero, for `nent` entries of pp parts into L-limb numbers. */
+int mdg_squeeze(int L, long long nent, int pp, const double *x, double *y, void *stream);
+
+/* The whole reference split path for L-limb n x n inputs, fused on the
+   device: split (split_kind 4 -> split4pd, 8 -> split8pd, 12 -> mixsplit2,
+   L == 2 only), limb convolution on FP64 MFMA, squeeze into L limbs.
+   Equal to the reference's C++/TCtest.cu:53-114 chain wherever the
+   reference's FP64 accumulation is exact (inputs of equal leading
+   exponent, positive entries: its own test domain). */
+int mdg_tc_gemm(int L, int m, int n, int k, const double *A, const double *B,
+                double *C, int split_kind, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Utilities.
+ * -------------------------------------------------------------------- */
+
+/* Seeded counter-based generator: `count` L-limb entries.
+   kind 0: hi ~ U[-1,1), lo = hi*2^-53*U[-0.5,0.5), QuickTwoSum(hi, lo)
+   kind 1: leading limb s*2^e, e uniform in [emin, emax], s ~ +-U[1,2);
+           next limb = previous * 2^-53 * U[-0.5,0.5)
+   kind 2: the reference's random_pd layout (C++/floatybits.cu:107-120):
+           positive limbs of 52 fraction bits at exponents hiexp - 52 i. */
+int mdg_fill_random(int kind, int L, long long count, unsigned long long seed,
+                    int emin, int emax, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDGEMM_H */

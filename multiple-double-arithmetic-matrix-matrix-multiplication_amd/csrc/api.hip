@@ -1,0 +1,315 @@
+// api.hip -- the extern "C" entry points of libmdgemm.so (include/mdgemm.h):
+// argument validation, status codes, scratch management and dispatch.
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace mdg {
+
+int vector_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                double *C, long long ldc, int twoprod, hipStream_t s);
+int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
+               long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s);
+int dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, hipStream_t s);
+int conv_gemm(int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C1q, hipStream_t s);
+int conv_squeeze(int L, int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C, hipStream_t s);
+int ref_split(int kind, const double *x, long long count, int p, double *out, hipStream_t s);
+int squeeze(int L, long long nent, int pp, const double *x, double *y, hipStream_t s);
+int bigB2(const double *Bq, int n, int pp, double *out, hipStream_t s);
+int fill_random(int kind, int L, long long count, unsigned long long seed, int emin, int emax, double *out,
+                hipStream_t s);
+int batched_gemm(int count, const mdg_problem *problems, int method, int target_bits, int spread_bits,
+                 int persistent, hipStream_t s);
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *get_error() { return g_err; }
+
+// ------------------------------------------------------------ scratch --
+struct ScratchBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+static std::mutex g_scratch_mu;
+static std::map<std::pair<int, void *>, std::vector<ScratchBuf>> g_scratch;
+
+void *scratch(void *stream, size_t bytes, int slot)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto &slots = g_scratch[{dev, stream}];
+    if ((int)slots.size() <= slot) slots.resize(slot + 1);
+    ScratchBuf &b = slots[slot];
+    if (bytes == 0) bytes = 1;
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            // the stream may still use the old buffer
+            if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return nullptr;
+            (void)hipFree(b.ptr);
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        // grow with headroom so alternating sizes do not reallocate
+        const size_t want = bytes + bytes / 8;
+        if (hipMalloc(&b.ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            b.ptr = nullptr;
+            return nullptr;
+        }
+        b.bytes = want;
+    }
+    return b.ptr;
+}
+
+static int release_scratch()
+{
+    int dev = 0;
+    MDG_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+        if (it->first.first == dev) {
+            MDG_HIP_CHECK(hipStreamSynchronize((hipStream_t)it->first.second));
+            for (auto &b : it->second)
+                if (b.ptr) (void)hipFree(b.ptr);
+            it = g_scratch.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return MDG_SUCCESS;
+}
+
+static int ensure_device()
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP device available (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+        return MDG_ERROR_HIP;
+    }
+    return MDG_SUCCESS;
+}
+
+static int check_gemm_args(int L, int m, int n, int k, const void *A, long long lda, const void *B, long long ldb,
+                           const void *C, long long ldc)
+{
+    MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d (expected 2, 4, 8 or 16)", L);
+    MDG_REQUIRE(m >= 0 && n >= 0 && k >= 0, MDG_ERROR_INVALID_VALUE, "negative dimension (m=%d n=%d k=%d)", m, n, k);
+    MDG_REQUIRE(lda >= (k > 0 ? k : 1) && ldb >= (n > 0 ? n : 1) && ldc >= (n > 0 ? n : 1), MDG_ERROR_INVALID_VALUE,
+                "leading dimension too small (lda=%lld >= k=%d, ldb=%lld >= n=%d, ldc=%lld >= n=%d)", lda, k, ldb, n,
+                ldc, n);
+    MDG_REQUIRE((m == 0 || n == 0) || C, MDG_ERROR_INVALID_VALUE, "C is NULL");
+    MDG_REQUIRE((m == 0 || n == 0 || k == 0) || (A && B), MDG_ERROR_INVALID_VALUE, "A or B is NULL");
+    return MDG_SUCCESS;
+}
+
+__global__ void zero_entries_kernel(double *C, int m, int n, long long ldc, int L)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * n * L) return;
+    const long long r = idx / ((long long)n * L);
+    const long long rem = idx - r * n * L;
+    C[r * ldc * L + rem] = 0.0;
+}
+
+static int zero_c(double *C, int m, int n, long long ldc, int L, hipStream_t s)
+{
+    const long long nel = (long long)m * n * L;
+    if (nel == 0) return MDG_SUCCESS;
+    hipLaunchKernelGGL(zero_entries_kernel, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, s, C, m, n, ldc, L);
+    MDG_LAUNCH_CHECK("zero_entries_kernel");
+    return MDG_SUCCESS;
+}
+
+}  // namespace mdg
+
+using namespace mdg;
+
+extern "C" {
+
+const char *mdg_last_error(void) { return get_error(); }
+
+int mdg_version(void) { return 100; }
+
+int mdg_init(void)
+{
+    int rc = ensure_device();
+    if (rc) return rc;
+    MDG_HIP_CHECK(hipFree(nullptr));
+    return MDG_SUCCESS;
+}
+
+int mdg_release_workspace(void) { return release_scratch(); }
+
+int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                    double *C, long long ldc, int twoprod, void *stream)
+{
+    int rc = check_gemm_args(L, m, n, k, A, lda, B, ldb, C, ldc);
+    if (rc) return rc;
+    MDG_REQUIRE(twoprod == MDG_TWOPROD_FMA || twoprod == MDG_TWOPROD_DEKKER, MDG_ERROR_INVALID_VALUE,
+                "unknown two_prod mode %d", twoprod);
+    if ((rc = ensure_device())) return rc;
+    if (m == 0 || n == 0) return MDG_SUCCESS;
+    if (k == 0) return zero_c(C, m, n, ldc, L, as_stream(stream));
+    return vector_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, twoprod, as_stream(stream));
+}
+
+int mdg_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
+             long long ldc, int method, int target_bits, mdg_gemm_info *info, void *stream)
+{
+    int rc = check_gemm_args(L, m, n, k, A, lda, B, ldb, C, ldc);
+    if (rc) return rc;
+    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_BF16_OZAKI, MDG_ERROR_INVALID_VALUE,
+                "unknown method %d", method);
+    if ((rc = ensure_device())) return rc;
+    if (info) {
+        *info = mdg_gemm_info{};
+        info->method = method;
+    }
+    if (m == 0 || n == 0) return MDG_SUCCESS;
+    if (k == 0) return zero_c(C, m, n, ldc, L, as_stream(stream));
+    if (method == MDG_METHOD_VECTOR)
+        return vector_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, MDG_TWOPROD_FMA, as_stream(stream));
+    return split_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, -1, info, as_stream(stream));
+}
+
+int mdg_gemm_batched(int count, const mdg_problem *problems, int method, int target_bits, int spread_bits,
+                     int persistent, void *stream)
+{
+    MDG_REQUIRE(count >= 0, MDG_ERROR_INVALID_VALUE, "negative batch count");
+    MDG_REQUIRE(count == 0 || problems, MDG_ERROR_INVALID_VALUE, "problems is NULL");
+    for (int i = 0; i < count; ++i) {
+        const mdg_problem &p = problems[i];
+        int rc = check_gemm_args(p.L, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc);
+        if (rc) {
+            std::string msg = std::string("problem ") + std::to_string(i) + ": " + get_error();
+            set_error("%s", msg.c_str());
+            return rc;
+        }
+    }
+    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_BF16_OZAKI, MDG_ERROR_INVALID_VALUE,
+                "unknown method %d", method);
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (count == 0) return MDG_SUCCESS;
+    return batched_gemm(count, problems, method, target_bits, spread_bits, persistent, as_stream(stream));
+}
+
+int mdg_split4pd(const double *x, long long count, int p, double *out, void *stream)
+{
+    MDG_REQUIRE(p >= 1 && count >= 0 && count % p == 0, MDG_ERROR_INVALID_VALUE,
+                "count=%lld must be a non-negative multiple of p=%d", count, p);
+    MDG_REQUIRE(count == 0 || (x && out), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return ref_split(4, x, count, p, out, as_stream(stream));
+}
+
+int mdg_split8pd(const double *x, long long count, int p, double *out, void *stream)
+{
+    MDG_REQUIRE(p >= 1 && count >= 0 && count % p == 0, MDG_ERROR_INVALID_VALUE,
+                "count=%lld must be a non-negative multiple of p=%d", count, p);
+    MDG_REQUIRE(count == 0 || (x && out), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return ref_split(8, x, count, p, out, as_stream(stream));
+}
+
+int mdg_mixsplit2(const double *x, long long count, double *out, void *stream)
+{
+    MDG_REQUIRE(count >= 0 && count % 2 == 0, MDG_ERROR_INVALID_VALUE, "count=%lld must be even", count);
+    MDG_REQUIRE(count == 0 || (x && out), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return ref_split(12, x, count, 2, out, as_stream(stream));
+}
+
+int mdg_bigB2(const double *Bq, int n, int pp, double *out, void *stream)
+{
+    MDG_REQUIRE(n >= 0 && pp >= 1, MDG_ERROR_INVALID_VALUE, "bad n=%d or pp=%d", n, pp);
+    MDG_REQUIRE(n == 0 || (Bq && out), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return bigB2(Bq, n, pp, out, as_stream(stream));
+}
+
+int mdg_dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, void *stream)
+{
+    MDG_REQUIRE(M >= 0 && N >= 0 && K >= 0, MDG_ERROR_INVALID_VALUE, "negative dimension");
+    MDG_REQUIRE(M == 0 || N == 0 || K == 0 || (A && B && C), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (M == 0 || N == 0 || K == 0) return MDG_SUCCESS;
+    return dgemm_acc(M, N, K, A, B, C, as_stream(stream));
+}
+
+int mdg_conv_gemm(int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C1q, void *stream)
+{
+    MDG_REQUIRE(m >= 0 && n >= 0 && k >= 1 && pp >= 1, MDG_ERROR_INVALID_VALUE, "bad shape m=%d n=%d k=%d pp=%d", m,
+                n, k, pp);
+    MDG_REQUIRE(pp <= 65535, MDG_ERROR_NOT_SUPPORTED, "pp=%d too large", pp);
+    MDG_REQUIRE(m == 0 || n == 0 || (Aq && Bq && C1q), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (m == 0 || n == 0) return MDG_SUCCESS;
+    return conv_gemm(m, n, k, pp, Aq, Bq, C1q, as_stream(stream));
+}
+
+int mdg_squeeze(int L, long long nent, int pp, const double *x, double *y, void *stream)
+{
+    MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d", L);
+    MDG_REQUIRE(nent >= 0 && pp >= 1, MDG_ERROR_INVALID_VALUE, "bad nent=%lld or pp=%d", nent, pp);
+    MDG_REQUIRE(nent == 0 || (x && y), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return squeeze(L, nent, pp, x, y, as_stream(stream));
+}
+
+int mdg_tc_gemm(int L, int m, int n, int k, const double *A, const double *B, double *C, int split_kind,
+                void *stream)
+{
+    MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d", L);
+    MDG_REQUIRE(split_kind == 4 || split_kind == 8 || (split_kind == 12 && L == 2), MDG_ERROR_INVALID_VALUE,
+                "split_kind must be 4, 8, or 12 (L == 2 only)");
+    MDG_REQUIRE(m >= 0 && n >= 0 && k >= 1, MDG_ERROR_INVALID_VALUE, "bad shape m=%d n=%d k=%d", m, n, k);
+    MDG_REQUIRE(m == 0 || n == 0 || (A && B && C), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (m == 0 || n == 0) return MDG_SUCCESS;
+    hipStream_t s = as_stream(stream);
+    const int per = (split_kind == 12) ? 6 : split_kind;
+    const int pp = per * L;
+    double *Aq = (double *)scratch(stream, sizeof(double) * (size_t)m * k * pp, 5);
+    double *Bq = (double *)scratch(stream, sizeof(double) * (size_t)k * n * pp, 6);
+    MDG_REQUIRE(Aq && Bq, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    if ((rc = ref_split(split_kind, A, (long long)m * k * L, L, Aq, s))) return rc;
+    if ((rc = ref_split(split_kind, B, (long long)k * n * L, L, Bq, s))) return rc;
+    return conv_squeeze(L, m, n, k, pp, Aq, Bq, C, s);
+}
+
+int mdg_fill_random(int kind, int L, long long count, unsigned long long seed, int emin, int emax, double *out,
+                    void *stream)
+{
+    MDG_REQUIRE(kind >= 0 && kind <= 2, MDG_ERROR_INVALID_VALUE, "unknown generator kind %d", kind);
+    MDG_REQUIRE(L >= 1 && L <= 64, MDG_ERROR_INVALID_VALUE, "bad limb count %d", L);
+    MDG_REQUIRE(count >= 0 && emax >= emin, MDG_ERROR_INVALID_VALUE, "bad count or exponent range");
+    MDG_REQUIRE(count == 0 || out, MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return fill_random(kind, L, count, seed, emin, emax, out, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// common.hpp -- status plumbing shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/mdgemm.h"
+
+namespace mdg {
+
+// thread-local text of the last failure (mdg_last_error)
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+struct Status {
+    int code;
+};
+
+#define MDG_HIP_CHECK(expr)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            ::mdg::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                  \
+            return MDG_ERROR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+#define MDG_REQUIRE(cond, code, ...)      \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::mdg::set_error(__VA_ARGS__); \
+            return (code);                \
+        }                                 \
+    } while (0)
+
+// check the launch that was just enqueued
+#define MDG_LAUNCH_CHECK(name)                                                            \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess) {                                                           \
+            ::mdg::set_error("launch of %s failed: %s", name, hipGetErrorString(_e));     \
+            return MDG_ERROR_HIP;                                                         \
+        }                                                                                 \
+    } while (0)
+
+inline bool valid_limbs(int L) { return L == 2 || L == 4 || L == 8 || L == 16; }
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device scratch, cached per (device, stream) and grown on demand.  Calls
+// that share a stream are ordered by it; different streams get different
+// buffers, so concurrent streams never alias scratch.
+void *scratch(void *stream, size_t bytes, int slot);
+
+// counter-based generator used by mdg_fill_random and the split kernels' tests
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace mdg

@@ -1,0 +1,899 @@
+// split_gemm.hip -- split-product GEMM on the MFMA cores.
+//
+// C = A * B for m x k times k x n matrices of L-limb numbers, as
+//   1. scan:        per row of A / column of B the scale exponent E (every
+//                   |entry| < 2^E) and an exponent-spread estimate;
+//   2. slice:       every entry cut into S balanced base-2^w digits of the
+//                   fixed-point integer floor(x * 2^(W-2-E))  (slices.hpp);
+//                   A slices land in A_all[r][t*kp + kk], B slices in
+//                   Bt_all[c][(S-1-u)*kp + kk]: both operands K-contiguous,
+//                   B's slices in reverse order;
+//   3. diagonal GEMMs: for every anti-diagonal g < S,
+//                   D_g = sum_{t+u=g} A_t * B_u
+//                       = A_all[:, 0:(g+1)kp] * Bt_all[:, (S-1-g)kp : S kp]^T,
+//                   one plain TN GEMM of depth (g+1)*kp per diagonal; all
+//                   diagonals in one launch (blockIdx.z), deepest first;
+//   4. reconstruct: C[r][c] = fold_g inc_d(C, D_g[r][c] * 2^(E_r+F_c+4-w(g+2)))
+//                   -- the TwoSum renormalisation cascade the reference's
+//                   squeeze uses (C++/TCcuda.cu:876-906).
+// This replaces the reference's split -> block-Toeplitz WMMA GEMM -> squeeze
+// chain (C++/floatybits.cu:159-342, C++/TCcuda.cu:79-95, C++/TCtest.cu:21-46,
+// C++/TCcuda.cu:876-958): the slice pairs with t+u <= S-1 are the non-zero
+// blocks of the reference's Toeplitz operand, never materialised here.
+//
+// Three GEMM back ends share the slicing:
+//   INT8  w = 8   v_mfma_i32_16x16x64_i8   exact INT32 sums, widened to
+//                                          INT64 in-loop when (g+1)kp*2^14
+//                                          could reach 2^31
+//   BF16  w = 8   v_mfma_f32_16x16x32_bf16 exact FP32 sums (< 2^24), widened
+//                                          to INT64 every 1024 k
+//   FP64  w <= 26 v_mfma_f64_16x16x4_f64   exact FP64 sums (< 2^53), widened
+//                                          to INT64 every 2^(55-2w) k
+#include <climits>
+#include <cmath>
+
+#include "common.hpp"
+#include "md_arith.hpp"
+#include "slices.hpp"
+
+namespace mdg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int ZERO_SCALE = -4000;   // scale exponent of an all-zero row/column
+
+// ------------------------------------------------------------------ scan --
+// stats[0] (int): max spread over rows/cols, stats[1] (int): non-finite flag
+__device__ __forceinline__ int scale_exponent(double s)
+{
+    // smallest E with s * (1 + 2^-40) < 2^E, s = sum of |limbs| of the entry
+    if (s == 0.0) return INT_MIN;
+    int ex;
+    frexp(s * (1.0 + 0x1p-40), &ex);
+    return ex;
+}
+
+// A rows: one wavefront per row
+template <int L>
+__global__ __launch_bounds__(256) void scan_rows_kernel(int m, int k, const double *__restrict__ A,
+                                                        long long lda, int *__restrict__ E,
+                                                        int *__restrict__ stats)
+{
+    const int lane = threadIdx.x & 63;
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= m) return;
+    int emax = INT_MIN;
+    double sum0 = 0.0;
+    int bad = 0;
+    for (int kk = lane; kk < k; kk += 64) {
+        const double *x = A + (r * lda + kk) * L;
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const double v = x[l];
+            bad |= !isfinite(v);
+            s += fabs(v);
+        }
+        emax = max(emax, scale_exponent(s));
+        sum0 += fabs(x[0]);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        emax = max(emax, __shfl_xor(emax, o));
+        sum0 += __shfl_xor(sum0, o);
+        bad |= __shfl_xor(bad, o);
+    }
+    if (lane == 0) {
+        E[r] = (emax == INT_MIN) ? ZERO_SCALE : emax;
+        if (emax != INT_MIN && sum0 > 0.0) {
+            // spread: E - log2(mean |x0|), rounded up, clamped to [0, 256]
+            const double sp = (double)emax - log2(sum0 / (double)k);
+            const int isp = (int)fmin(256.0, fmax(0.0, ceil(sp)));
+            atomicMax(&stats[0], isp);
+        }
+        if (bad) atomicOr(&stats[1], 1);
+    }
+}
+
+// B columns: 64 columns x 4 row groups per block, rows chunked over grid.y;
+// partial results meet in E (atomicMax, pre-filled with INT_MIN) and in
+// colsum (atomicAdd, pre-filled with 0).
+template <int L>
+__global__ __launch_bounds__(256) void scan_cols_kernel(int k, int n, const double *__restrict__ B,
+                                                        long long ldb, int rows_per_block,
+                                                        int *__restrict__ E, double *__restrict__ colsum,
+                                                        int *__restrict__ stats)
+{
+    __shared__ int se[4][64];
+    __shared__ double ss[4][64];
+    const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const long long c = (long long)blockIdx.x * 64 + cl;
+    const int k0 = blockIdx.y * rows_per_block;
+    const int k1 = min(k, k0 + rows_per_block);
+    int emax = INT_MIN;
+    double sum0 = 0.0;
+    int bad = 0;
+    if (c < n) {
+        for (int kk = k0 + grp; kk < k1; kk += 4) {
+            const double *x = B + ((long long)kk * ldb + c) * L;
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const double v = x[l];
+                bad |= !isfinite(v);
+                s += fabs(v);
+            }
+            emax = max(emax, scale_exponent(s));
+            sum0 += fabs(x[0]);
+        }
+    }
+    se[grp][cl] = emax;
+    ss[grp][cl] = sum0;
+    if (bad) atomicOr(&stats[1], 1);
+    __syncthreads();
+    if (grp == 0 && c < n) {
+        int e = max(max(se[0][cl], se[1][cl]), max(se[2][cl], se[3][cl]));
+        double s = (ss[0][cl] + ss[1][cl]) + (ss[2][cl] + ss[3][cl]);
+        if (e != INT_MIN) atomicMax(&E[c], e);
+        if (s != 0.0) atomicAdd(&colsum[c], s);
+    }
+}
+
+// finish the column scan: zero columns get ZERO_SCALE, spread into stats[2]
+__global__ void finish_cols_kernel(int k, int n, int *__restrict__ E,
+                                   const double *__restrict__ colsum, int *__restrict__ stats)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int e = E[c];
+    if (e == INT_MIN) {
+        E[c] = ZERO_SCALE;
+        return;
+    }
+    const double s = colsum[c];
+    if (s > 0.0) {
+        const double sp = (double)e - log2(s / (double)k);
+        const int isp = (int)fmin(256.0, fmax(0.0, ceil(sp)));
+        atomicMax(&stats[2], isp);
+    }
+}
+
+__global__ void fill_int_kernel(int *p, long long nel, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nel) p[i] = v;
+}
+
+__global__ void fill_double_kernel(double *p, long long nel, double v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nel) p[i] = v;
+}
+
+// ----------------------------------------------------------------- slice --
+template <typename T>
+__device__ __forceinline__ T digit_as(int d);
+template <>
+__device__ __forceinline__ int8_t digit_as<int8_t>(int d) { return (int8_t)d; }
+template <>
+__device__ __forceinline__ __bf16 digit_as<__bf16>(int d) { return (__bf16)(float)d; }
+template <>
+__device__ __forceinline__ int32_t digit_as<int32_t>(int d) { return d; }
+
+// A: thread per entry (r, kk) of the padded mp x kp domain, kk fastest.
+template <int L, typename T>
+__global__ __launch_bounds__(256) void slice_a_kernel(int m, int k, int mp, int kp, const double *__restrict__ A,
+                                                      long long lda, const int *__restrict__ E, int S, int w,
+                                                      T *__restrict__ out)
+{
+    constexpr int NW = FixedWords<L>::NW;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)mp * kp) return;
+    const long long r = idx / kp;
+    const int kk = (int)(idx - r * kp);
+    T *o = out + r * ((long long)S * kp) + kk;
+    if (r >= m || kk >= k || E[r] == ZERO_SCALE) {
+        for (int t = 0; t < S; ++t) o[(long long)t * kp] = digit_as<T>(0);
+        return;
+    }
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = A[(r * lda + kk) * L + l];
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, S * w - 2 - E[r], N);
+    fixed_digits<NW>(N, S, w, [&](int t, int d) { o[(long long)t * kp] = digit_as<T>(d); });
+}
+
+// B: thread per entry (kk, c) of the padded kp x np domain, kk fastest so
+// the transposed, slice-reversed writes Bt[c][(S-1-u)*kp + kk] coalesce.
+template <int L, typename T>
+__global__ __launch_bounds__(256) void slice_b_kernel(int k, int n, int kp, int np, const double *__restrict__ B,
+                                                      long long ldb, const int *__restrict__ F, int S, int w,
+                                                      T *__restrict__ out)
+{
+    constexpr int NW = FixedWords<L>::NW;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)np * kp) return;
+    const long long c = idx / kp;
+    const int kk = (int)(idx - c * kp);
+    T *o = out + c * ((long long)S * kp) + kk;
+    if (c >= n || kk >= k || F[c] == ZERO_SCALE) {
+        for (int u = 0; u < S; ++u) o[(long long)u * kp] = digit_as<T>(0);
+        return;
+    }
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = B[((long long)kk * ldb + c) * L + l];
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, S * w - 2 - F[c], N);
+    fixed_digits<NW>(N, S, w, [&](int u, int d) { o[(long long)(S - 1 - u) * kp] = digit_as<T>(d); });
+}
+
+// ------------------------------------------------------ tile scheduling --
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Give every XCD a contiguous block of tiles and walk that block in
+// groups of GROUP tile-rows, so the tiles resident on one XCD share A rows
+// and B columns in its L2.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    const int T = tiles_m * tiles_n;
+    int lin = bid;
+    if ((T & 7) == 0) lin = (bid & 7) * (T >> 3) + (bid >> 3);
+    constexpr int GROUP = 8;
+    const int per_group = GROUP * tiles_n;
+    const int gidx = lin / per_group;
+    const int first_m = gidx * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int in = lin - gidx * per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+// --------------------------------------------------- INT8 diagonal GEMM --
+// Workgroup tile 128 x 128, 4 waves in 2 x 2, each 64 x 64 = 4 x 4 MFMA
+// 16x16x64 tiles; k-block of 128 bytes staged in LDS with rows padded to
+// 144 bytes (conflict-free 16-byte fragment reads).  Buffers are padded so
+// no bounds checks are needed inside the K loop.
+template <bool WIDE>
+__global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__ Aall,
+                                                      const int8_t *__restrict__ Ball, int S, int kp,
+                                                      int m, int n, int tiles_m, int tiles_n,
+                                                      int flush_blocks, void *__restrict__ Dout)
+{
+    constexpr int BM = 128, BN = 128, BK = 128, LDR = BK + 16;
+    __shared__ __attribute__((aligned(16))) int8_t As[BM * LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[BN * LDR];
+
+    const int g = S - 1 - (int)blockIdx.z;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const long long ld = (long long)S * kp;
+    const int8_t *Ab = Aall + (long long)tm * BM * ld;
+    const int8_t *Bb = Ball + (long long)tn * BN * ld + (long long)(S - 1 - g) * kp;
+    const int Kd = (g + 1) * kp;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = lane & 15, lk = (lane >> 4) * 16;
+
+    i4 acc[4][4];
+    long long wide[4][4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = (i4){0, 0, 0, 0};
+            if constexpr (WIDE) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
+            }
+        }
+
+    int blk = 0;
+    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            const int row = idx >> 3, seg = (idx & 7) * 16;
+            *(uint4 *)&As[row * LDR + seg] = *(const uint4 *)(Ab + (long long)row * ld + kb + seg);
+            *(uint4 *)&Bs[row * LDR + seg] = *(const uint4 *)(Bb + (long long)row * ld + kb + seg);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 64) {
+            i4 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = *(const i4 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = *(const i4 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if constexpr (WIDE) {
+            if ((blk + 1) % flush_blocks == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) wide[i][j][v] += acc[i][j][v];
+                        acc[i][j] = (i4){0, 0, 0, 0};
+                    }
+            }
+        }
+    }
+    // store D_g: acc element v of tile (i, j) is row 4*(lane/16)+v, col lane%16
+    const long long plane = (long long)g * m * n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long col = (long long)tn * BN + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long row = (long long)tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                if (row < m && col < n) {
+                    if constexpr (WIDE)
+                        ((long long *)Dout)[plane + row * n + col] = wide[i][j][v] + acc[i][j][v];
+                    else
+                        ((int *)Dout)[plane + row * n + col] = acc[i][j][v];
+                }
+            }
+        }
+}
+
+// --------------------------------------------------- BF16 diagonal GEMM --
+// Same tiling as INT8 with 64 bf16 (128 bytes) per k-block row and the
+// 16x16x32 bf16 MFMA; FP32 sums stay exact below 2^24, so every 16 k-blocks
+// (1024 products of at most 2^14) they are moved into INT64.
+__global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict__ Aall,
+                                                        const __bf16 *__restrict__ Ball, int S, int kp,
+                                                        int m, int n, int tiles_m, int tiles_n,
+                                                        long long *__restrict__ Dout)
+{
+    constexpr int BM = 128, BN = 128, BK = 64, LDR = BK + 8;   // elements
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM * LDR];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * LDR];
+
+    const int g = S - 1 - (int)blockIdx.z;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const long long ld = (long long)S * kp;
+    const __bf16 *Ab = Aall + (long long)tm * BM * ld;
+    const __bf16 *Bb = Ball + (long long)tn * BN * ld + (long long)(S - 1 - g) * kp;
+    const int Kd = (g + 1) * kp;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = lane & 15, lk = (lane >> 4) * 8;
+
+    f4 acc[4][4];
+    long long wide[4][4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
+        }
+
+    int blk = 0;
+    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            const int row = idx >> 3, seg = (idx & 7) * 8;
+            *(uint4 *)&As[row * LDR + seg] = *(const uint4 *)(Ab + (long long)row * ld + kb + seg);
+            *(uint4 *)&Bs[row * LDR + seg] = *(const uint4 *)(Bb + (long long)row * ld + kb + seg);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
+            bf8 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = *(const bf8 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = *(const bf8 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (((blk + 1) & 15) == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) wide[i][j][v] += (long long)acc[i][j][v];
+                    acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+                }
+        }
+    }
+    const long long plane = (long long)g * m * n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long col = (long long)tn * BN + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long row = (long long)tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                if (row < m && col < n) Dout[plane + row * n + col] = wide[i][j][v] + (long long)acc[i][j][v];
+            }
+        }
+}
+
+// --------------------------------------------------- FP64 diagonal GEMM --
+// Workgroup tile 128 x 64, 4 waves in 2 x 2, each 64 x 32 = 4 x 2 MFMA
+// 16x16x4 tiles; k-block of 16 staged in LDS as doubles (converted from the
+// INT32 slices on the way in, halving the global traffic).
+//   OUT 0: store FP64 sums          (limb convolution of the reference path)
+//   OUT 1: add FP64 sums into C     (the reference's accumulate GEMM)
+//   OUT 2: store INT64, widening every flush_blocks k-blocks (split path)
+// TA is the stored slice type (int32_t or double).  All loads are
+// bounds-checked against (rows, Kd) so unpadded operands work.
+template <typename TA, int OUT>
+__global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aall, const TA *__restrict__ Ball,
+                                                       int S, int kp, long long lda, long long ldb, int m, int n,
+                                                       int tiles_m, int tiles_n, int flush_blocks,
+                                                       void *__restrict__ Dout, long long ldd)
+{
+    constexpr int BM = 128, BN = 64, BK = 16, LDR = BK + 1;
+    __shared__ double As[BM * LDR];
+    __shared__ double Bs[BN * LDR];
+
+    const int g = S - 1 - (int)blockIdx.z;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const long long row0 = (long long)tm * BM, col0 = (long long)tn * BN;
+    const TA *Ab = Aall + row0 * lda;
+    const TA *Bb = Ball + col0 * ldb + (long long)(S - 1 - g) * kp;
+    const int Kd = (g + 1) * kp;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = lane & 15, lk = lane >> 4;
+
+    d4 acc[4][2];
+    long long wide[4][2][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
+        }
+
+    int blk = 0;
+    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
+        // A: 128 rows x 16 k = 2048 values, 8 per thread; B: 64 x 16 = 1024, 4 per thread
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int idx = tid + it * 256;
+            const int row = idx >> 4, kk = idx & 15;
+            double v = 0.0;
+            if (row0 + row < m && kb + kk < Kd) v = (double)Ab[(long long)row * lda + kb + kk];
+            As[row * LDR + kk] = v;
+        }
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            const int row = idx >> 4, kk = idx & 15;
+            double v = 0.0;
+            if (col0 + row < n && kb + kk < Kd) v = (double)Bb[(long long)row * ldb + kb + kk];
+            Bs[row * LDR + kk] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double a[4], b[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[(wn * 32 + j * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if constexpr (OUT == 2) {
+            if ((blk + 1) % flush_blocks == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) wide[i][j][v] += (long long)acc[i][j][v];
+                        acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+                    }
+            }
+        }
+    }
+    const long long plane = (long long)g * m * ldd;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const long long col = col0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long row = row0 + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                if (row < m && col < n) {
+                    const long long o = plane + row * ldd + col;
+                    if constexpr (OUT == 0)
+                        ((double *)Dout)[o] = acc[i][j][v];
+                    else if constexpr (OUT == 1)
+                        ((double *)Dout)[o] += acc[i][j][v];
+                    else
+                        ((long long *)Dout)[o] = wide[i][j][v] + (long long)acc[i][j][v];
+                }
+            }
+        }
+}
+
+// ------------------------------------------------------------ reconstruct --
+// C[r][c] = fold over g of inc_d(C, D_g[r][c] * 2^(E_r + F_c + 4 - w(g+2))),
+// starting from zero, g = 0 (most significant) first.  An INT64 diagonal
+// sum is split into two exact doubles (high 32-bit-aligned part, low part).
+template <int L, typename PT>
+__global__ __launch_bounds__(256) void reconstruct_kernel(int m, int n, int S, int w, const PT *__restrict__ D,
+                                                          const int *__restrict__ E, const int *__restrict__ F,
+                                                          double *__restrict__ C, long long ldc)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * n) return;
+    const long long r = idx / n, c = idx - r * n;
+    double y[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    const int er = E[r], fc = F[c];
+    if (er != ZERO_SCALE && fc != ZERO_SCALE) {
+        const long long mn = (long long)m * n;
+        for (int g = 0; g < S; ++g) {
+            const int sc = er + fc + 4 - w * (g + 2);
+            const PT v = D[(long long)g * mn + idx];
+            if constexpr (sizeof(PT) == 8) {
+                const long long hi = v & ~0xFFFFFFFFll;
+                const long long lo = v - hi;   // in [0, 2^32)
+                if (hi != 0) md_inc_d<L>(y, ldexp((double)hi, sc));
+                if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
+            } else {
+                if (v != 0) md_inc_d<L>(y, ldexp((double)v, sc));
+            }
+        }
+    }
+    double *o = C + (r * ldc + c) * L;
+#pragma unroll
+    for (int l = 0; l < L; ++l) o[l] = y[l];
+}
+
+// ---------------------------------------------------------------- driver --
+static inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
+
+template <typename K>
+static inline unsigned blocks_for(long long nel, int bs) { return (unsigned)((nel + bs - 1) / bs); }
+
+struct Plan {
+    int method, L, m, n, k;
+    int S, w, W;
+    int kp, mp, np;
+    bool wide;        // INT8: widened accumulation
+    int flush_blocks; // FP64 / INT8
+    int spread_a, spread_b;
+};
+
+// Choose slice width and count for a precision target and the measured
+// spreads (see slices.hpp for the error terms): the dominant error of the
+// product is (S+1) * L * 2^(2 - W + spread_a + spread_b) relative to
+// sum_k |a||b|, so W = target + spreads + log2((S+1) L) + 2.
+static int plan_slices(Plan &p, int target_bits)
+{
+    const int base = target_bits + p.spread_a + p.spread_b + 2;
+    auto need_W = [&](int S) {
+        const double extra = std::ceil(std::log2((double)(S + 1) * p.L));
+        return base + (int)extra;
+    };
+    if (p.method == MDG_METHOD_INT8_OZAKI || p.method == MDG_METHOD_BF16_OZAKI) {
+        p.w = 8;
+        int S = 4;
+        while (S * 8 < need_W(S)) ++S;
+        p.S = S;
+    } else {
+        // FP64: widest w with (2w-2) + log2(S * kp) <= 62 (INT64 diagonal
+        // sums) and one 16-deep k-block exact in FP64 (16 * 2^(2w-2) <= 2^53)
+        p.w = 0;
+        for (int w = 25; w >= 8; --w) {
+            int S = 2;
+            while (S * w < need_W(S)) ++S;
+            const double bits = (2.0 * w - 2.0) + std::log2((double)S * p.kp);
+            if (bits <= 62.0) {
+                p.w = w;
+                p.S = S;
+                break;
+            }
+        }
+        if (p.w == 0) return MDG_ERROR_RANGE;
+        // FP64 partial sums exact while count * 2^(2w-2) <= 2^53
+        const long long chunk = 1ll << (55 - 2 * p.w);
+        p.flush_blocks = (int)std::max(1ll, chunk / 16);
+    }
+    p.W = p.S * p.w;
+    return MDG_SUCCESS;
+}
+
+template <int L>
+static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const double *A, long long lda,
+                          const double *B, long long ldb, double *C, long long ldc, hipStream_t s,
+                          mdg_gemm_info *info)
+{
+    const int m = p.m, n = p.n, k = p.k;
+    // scratch: E (m), F (n), colsum (n), stats (4)
+    int *E = (int *)scratch(s, sizeof(int) * (size_t)(m + n + 8), 0);
+    MDG_REQUIRE(E, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    int *F = E + m;
+    int *stats = F + n;
+    double *colsum = (double *)scratch(s, sizeof(double) * (size_t)n, 1);
+    MDG_REQUIRE(colsum, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, F, (long long)n, INT_MIN);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, stats, 8ll, 0);
+    hipLaunchKernelGGL(fill_double_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, colsum, (long long)n, 0.0);
+    hipLaunchKernelGGL((scan_rows_kernel<L>), dim3((m + 3) / 4), dim3(256), 0, s, m, k, A, lda, E, stats);
+    const int rpb = 256;
+    hipLaunchKernelGGL((scan_cols_kernel<L>), dim3((n + 63) / 64, (k + rpb - 1) / rpb), dim3(256), 0, s, k, n, B,
+                       ldb, rpb, F, colsum, stats);
+    hipLaunchKernelGGL(finish_cols_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, k, n, F, colsum, stats);
+    MDG_LAUNCH_CHECK("scan kernels");
+    if (fixed_spread < 0) {
+        int hstats[4];
+        MDG_HIP_CHECK(hipMemcpyAsync(hstats, stats, sizeof(hstats), hipMemcpyDeviceToHost, s));
+        MDG_HIP_CHECK(hipStreamSynchronize(s));
+        MDG_REQUIRE(hstats[1] == 0, MDG_ERROR_NON_FINITE, "input contains Inf or NaN limbs");
+        p.spread_a = hstats[0];
+        p.spread_b = hstats[2];
+    } else {
+        // caller-supplied allowance: no read-back, the stream stays asynchronous
+        p.spread_a = fixed_spread;
+        p.spread_b = fixed_spread;
+    }
+
+    p.kp = (int)round_up(k, 128);
+    p.mp = (int)round_up(m, 128);
+    p.np = (int)round_up(n, 128);
+    int rc = plan_slices(p, target_bits);
+    if (rc) {
+        set_error("no slice width fits k=%d at %d target bits", k, target_bits);
+        return rc;
+    }
+    MDG_REQUIRE(p.W <= 32 * FixedWords<L>::NW - 1, MDG_ERROR_RANGE,
+                "needs %d fixed-point bits, more than the %d this limb count holds", p.W,
+                32 * FixedWords<L>::NW - 1);
+    const int S = p.S, w = p.w, kp = p.kp, mp = p.mp, np = p.np;
+    const long long mn = (long long)m * n;
+    const int tiles_m = mp / 128;
+    const unsigned zdim = (unsigned)S;
+    MDG_REQUIRE(zdim <= 65535u, MDG_ERROR_RANGE, "too many slices (%d)", S);
+
+    if (p.method == MDG_METHOD_INT8_OZAKI) {
+        // exact INT32 while (g+1)*kp * 2^14 < 2^31
+        p.wide = ((long long)S * kp > 131072ll - 128);
+        p.flush_blocks = (int)((131072ll - 128) / 128);
+        int8_t *Aall = (int8_t *)scratch(s, (size_t)mp * S * kp, 2);
+        int8_t *Ball = (int8_t *)scratch(s, (size_t)np * S * kp, 3);
+        void *D = scratch(s, (size_t)S * mn * (p.wide ? 8 : 4), 4);
+        MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+        hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp * kp, 256)), dim3(256), 0,
+                           s, m, k, mp, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
+                           s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        const int tiles_n = np / 128;
+        dim3 grid(tiles_m * tiles_n, 1, zdim);
+        if (p.wide)
+            hipLaunchKernelGGL((diag_i8_kernel<true>), grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m, tiles_n,
+                               p.flush_blocks, D);
+        else
+            hipLaunchKernelGGL((diag_i8_kernel<false>), grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m,
+                               tiles_n, p.flush_blocks, D);
+        MDG_LAUNCH_CHECK("diag_i8_kernel");
+        if (p.wide)
+            hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m,
+                               n, S, w, (const long long *)D, E, F, C, ldc);
+        else
+            hipLaunchKernelGGL((reconstruct_kernel<L, int>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S,
+                               w, (const int *)D, E, F, C, ldc);
+    } else if (p.method == MDG_METHOD_BF16_OZAKI) {
+        p.wide = true;
+        p.flush_blocks = 16;
+        __bf16 *Aall = (__bf16 *)scratch(s, (size_t)mp * S * kp * 2, 2);
+        __bf16 *Ball = (__bf16 *)scratch(s, (size_t)np * S * kp * 2, 3);
+        long long *D = (long long *)scratch(s, (size_t)S * mn * 8, 4);
+        MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+        hipLaunchKernelGGL((slice_a_kernel<L, __bf16>), dim3(blocks_for<int>((long long)mp * kp, 256)), dim3(256), 0,
+                           s, m, k, mp, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_b_kernel<L, __bf16>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
+                           s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        const int tiles_n = np / 128;
+        dim3 grid(tiles_m * tiles_n, 1, zdim);
+        hipLaunchKernelGGL(diag_bf16_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m, tiles_n, D);
+        MDG_LAUNCH_CHECK("diag_bf16_kernel");
+        hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
+                           S, w, (const long long *)D, E, F, C, ldc);
+    } else {
+        p.wide = true;
+        int32_t *Aall = (int32_t *)scratch(s, (size_t)mp * S * kp * 4, 2);
+        int32_t *Ball = (int32_t *)scratch(s, (size_t)np * S * kp * 4, 3);
+        long long *D = (long long *)scratch(s, (size_t)S * mn * 8, 4);
+        MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+        hipLaunchKernelGGL((slice_a_kernel<L, int32_t>), dim3(blocks_for<int>((long long)mp * kp, 256)), dim3(256), 0,
+                           s, m, k, mp, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_b_kernel<L, int32_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
+                           s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        const int tiles_n = np / 64;
+        dim3 grid(tiles_m * tiles_n, 1, zdim);
+        hipLaunchKernelGGL((diag_f64_kernel<int32_t, 2>), grid, dim3(256), 0, s, Aall, Ball, S, kp,
+                           (long long)S * kp, (long long)S * kp, m, n, tiles_m, tiles_n, p.flush_blocks, (void *)D,
+                           (long long)n);
+        MDG_LAUNCH_CHECK("diag_f64_kernel");
+        hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
+                           S, w, (const long long *)D, E, F, C, ldc);
+    }
+    MDG_LAUNCH_CHECK("split gemm");
+    if (info) {
+        info->method = p.method;
+        info->slices = p.S;
+        info->slice_bits = p.w;
+        info->diagonals = p.S;
+        info->pair_gemms = (long long)p.S * (p.S + 1) / 2;
+        info->total_bits = p.W;
+        info->spread_a = p.spread_a;
+        info->spread_b = p.spread_b;
+        info->flushed = p.wide ? 1 : 0;
+    }
+    return MDG_SUCCESS;
+}
+
+int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
+               long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s)
+{
+    Plan p{};
+    p.method = method;
+    p.L = L;
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    if (target_bits <= 0) target_bits = 53 * L;
+    switch (L) {
+    case 2: return run_split_gemm<2>(p, target_bits, fixed_spread, A, lda, B, ldb, C, ldc, s, info);
+    case 4: return run_split_gemm<4>(p, target_bits, fixed_spread, A, lda, B, ldb, C, ldc, s, info);
+    case 8: return run_split_gemm<8>(p, target_bits, fixed_spread, A, lda, B, ldb, C, ldc, s, info);
+    case 16: return run_split_gemm<16>(p, target_bits, fixed_spread, A, lda, B, ldb, C, ldc, s, info);
+    }
+    set_error("unsupported limb count L=%d", L);
+    return MDG_ERROR_INVALID_VALUE;
+}
+
+// ------------------------------------------- reference split-path pieces --
+// The generic FP64 TN kernel serves the reference's accumulate GEMM
+// (C++/TCtest.cu:21-46) and the limb convolution (Aq * bigB2(Bq)).
+
+int dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, hipStream_t s)
+{
+    const int tiles_m = (M + 127) / 128, tiles_n = (N + 63) / 64;
+    dim3 grid(tiles_m * tiles_n, 1, 1);
+    // one "diagonal" (S = 1) of depth K; B is column-major = K-contiguous rows
+    hipLaunchKernelGGL((diag_f64_kernel<double, 1>), grid, dim3(256), 0, s, A, B, 1, K, (long long)K, (long long)K, M,
+                       N, tiles_m, tiles_n, 1, (void *)C, (long long)N);
+    MDG_LAUNCH_CHECK("diag_f64_kernel (dgemm_acc)");
+    return MDG_SUCCESS;
+}
+
+// interleaved parts Xq[r][kk][t] (rows x k x pp) -> planes P[r][t*k + kk],
+// reversed (t -> pp-1-t) when `rev`; transposed source (Bq[kk][c][t]) when `trans`
+__global__ void parts_to_planes_kernel(int rows, int k, int pp, const double *__restrict__ Xq, long long row_stride,
+                                       long long k_stride, int rev, double *__restrict__ P)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)rows * k * pp) return;
+    const long long r = idx / ((long long)k * pp);
+    const long long rem = idx - r * k * pp;
+    const int t = (int)(rem / k);
+    const int kk = (int)(rem - (long long)t * k);
+    const int src_t = rev ? (pp - 1 - t) : t;
+    P[r * ((long long)pp * k) + (long long)t * k + kk] = Xq[r * row_stride + kk * k_stride + src_t];
+}
+
+// D planes (pp x m x n) -> C1q[r][c][i]
+__global__ void planes_to_parts_kernel(int m, int n, int pp, const double *__restrict__ D, double *__restrict__ C1q)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * n * pp) return;
+    const int i = (int)(idx % pp);
+    const long long rc = idx / pp;
+    C1q[idx] = D[(long long)i * m * n + rc];
+}
+
+// squeeze D planes directly: y[r][c] = fold_i inc_d(y, D_i[r][c])
+template <int L>
+__global__ __launch_bounds__(256) void squeeze_planes_kernel(long long nent, int pp, const double *__restrict__ D,
+                                                             double *__restrict__ Y)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nent) return;
+    double y[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    for (int i = 0; i < pp; ++i) md_inc_d<L>(y, D[(long long)i * nent + idx]);
+#pragma unroll
+    for (int l = 0; l < L; ++l) Y[idx * L + l] = y[l];
+}
+
+// C1q (or planes) = Aq * bigB2(Bq) for m x k and k x n matrices of pp parts.
+static int conv_planes(int m, int n, int k, int pp, const double *Aq, const double *Bq, double *D, hipStream_t s)
+{
+    double *Ap = (double *)scratch(s, sizeof(double) * (size_t)m * pp * k, 2);
+    double *Bp = (double *)scratch(s, sizeof(double) * (size_t)n * pp * k, 3);
+    MDG_REQUIRE(Ap && Bp, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    const long long na = (long long)m * k * pp, nb = (long long)n * k * pp;
+    hipLaunchKernelGGL(parts_to_planes_kernel, dim3(blocks_for<int>(na, 256)), dim3(256), 0, s, m, k, pp, Aq,
+                       (long long)k * pp, (long long)pp, 0, Ap);
+    // Bq[kk][c][t]: "row" c has stride pp, kk has stride n*pp
+    hipLaunchKernelGGL(parts_to_planes_kernel, dim3(blocks_for<int>(nb, 256)), dim3(256), 0, s, n, k, pp, Bq,
+                       (long long)pp, (long long)n * pp, 1, Bp);
+    const int tiles_m = (m + 127) / 128, tiles_n = (n + 63) / 64;
+    dim3 grid(tiles_m * tiles_n, 1, (unsigned)pp);
+    hipLaunchKernelGGL((diag_f64_kernel<double, 0>), grid, dim3(256), 0, s, Ap, Bp, pp, k, (long long)pp * k,
+                       (long long)pp * k, m, n, tiles_m, tiles_n, 1, (void *)D, (long long)n);
+    MDG_LAUNCH_CHECK("diag_f64_kernel (conv)");
+    return MDG_SUCCESS;
+}
+
+int conv_gemm(int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C1q, hipStream_t s)
+{
+    double *D = (double *)scratch(s, sizeof(double) * (size_t)pp * m * n, 4);
+    MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    int rc = conv_planes(m, n, k, pp, Aq, Bq, D, s);
+    if (rc) return rc;
+    const long long nel = (long long)m * n * pp;
+    hipLaunchKernelGGL(planes_to_parts_kernel, dim3(blocks_for<int>(nel, 256)), dim3(256), 0, s, m, n, pp, D, C1q);
+    MDG_LAUNCH_CHECK("planes_to_parts_kernel");
+    return MDG_SUCCESS;
+}
+
+int conv_squeeze(int L, int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C, hipStream_t s)
+{
+    double *D = (double *)scratch(s, sizeof(double) * (size_t)pp * m * n, 4);
+    MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    int rc = conv_planes(m, n, k, pp, Aq, Bq, D, s);
+    if (rc) return rc;
+    const long long nent = (long long)m * n;
+    const dim3 grid(blocks_for<int>(nent, 256));
+    switch (L) {
+    case 2: hipLaunchKernelGGL(squeeze_planes_kernel<2>, grid, dim3(256), 0, s, nent, pp, D, C); break;
+    case 4: hipLaunchKernelGGL(squeeze_planes_kernel<4>, grid, dim3(256), 0, s, nent, pp, D, C); break;
+    case 8: hipLaunchKernelGGL(squeeze_planes_kernel<8>, grid, dim3(256), 0, s, nent, pp, D, C); break;
+    case 16: hipLaunchKernelGGL(squeeze_planes_kernel<16>, grid, dim3(256), 0, s, nent, pp, D, C); break;
+    default: set_error("unsupported limb count L=%d", L); return MDG_ERROR_INVALID_VALUE;
+    }
+    MDG_LAUNCH_CHECK("squeeze_planes_kernel");
+    return MDG_SUCCESS;
+}
+
+}  // namespace mdg

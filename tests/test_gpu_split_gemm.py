@@ -1,0 +1,101 @@
+"""Split-product GEMMs (FP64 / INT8 / BF16 MFMA) against the exact checker:
+normwise error || C_exact - C || / || |A||B| || within c*k*2^(-53L), and
+agreement with the vector GEMM (the reference's arithmetic)."""
+import numpy as np
+import pytest
+
+from conftest import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["f64", "int8", "bf16"]
+
+
+def _bound(L, k):
+    # c * k * 2^(-53 L) with c = 4
+    return 4.0 * k * 2.0 ** (-53 * L)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("L", [2, 4, 8, 16])
+def test_split_gemm_error_bound(mdg, orc, torch_cuda, method, L):
+    for (m, n, k, emin, emax) in ((37, 29, 53, -8, 8), (64, 128, 96, -30, 30), (5, 3, 200, -2, 2)):
+        print(f"split {method} L={L} {m}x{n}x{k} e=[{emin},{emax}]", flush=True)
+        A = orc.generate(orc.GEN_QD, L, m, k, 7 * L + m, emin, emax)
+        B = orc.generate(orc.GEN_QD, L, k, n, 9 * L + n, emin, emax)
+        C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method=method)
+        C = to_host(C)
+        err, _ = orc.normwise_error(L, m, n, k, A, B, C)
+        assert err <= _bound(L, k), f"{method} L={L}: err={err:.3e} bound={_bound(L, k):.3e} {info}"
+        assert info["slices"] >= 1 and info["pair_gemms"] == info["slices"] * (info["slices"] + 1) // 2
+        V = orc.gemm(L, m, n, k, A, B, mode=orc.TP_FMA)
+        errv, _ = orc.normwise_error(L, m, n, k, A, B, V)
+        assert errv <= _bound(L, k)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_split_gemm_dd_config1_inputs(mdg, orc, torch_cuda, method):
+    # config 1's generator (hi ~ U[-1,1], lo = hi 2^-53 U[-.5,.5]) at n = 128
+    L, n = 2, 128
+    A = orc.generate(orc.GEN_DD, L, n, n, 1)
+    B = orc.generate(orc.GEN_DD, L, n, n, 2)
+    C, info = mdg.gemm(L, n, n, n, to_dev(A), to_dev(B), method=method)
+    err, _ = orc.normwise_error(L, n, n, n, A, B, to_host(C))
+    assert err <= _bound(L, n), (err, info)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_split_gemm_zero_rows_and_columns(mdg, orc, torch_cuda, method):
+    L, m, n, k = 4, 20, 18, 33
+    A = orc.generate(orc.GEN_QD, L, m, k, 3, -8, 8).reshape(m, k, L)
+    B = orc.generate(orc.GEN_QD, L, k, n, 4, -8, 8).reshape(k, n, L)
+    A[3] = 0.0
+    B[:, 5] = 0.0
+    C, _ = mdg.gemm(L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method=method)
+    C = to_host(C).reshape(m, n, L)
+    assert np.all(C[3] == 0.0) and np.all(C[:, 5] == 0.0)
+    err, _ = orc.normwise_error(L, m, n, k, A.ravel(), B.ravel(), C.ravel())
+    assert err <= _bound(L, k)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_split_gemm_all_zero(mdg, torch_cuda, method):
+    L, m, n, k = 2, 9, 7, 5
+    z = to_dev(np.zeros(max(m, n) * k * L))
+    C, _ = mdg.gemm(L, m, n, k, z, z, method=method, C=to_dev(np.ones(m * n * L)))
+    assert np.all(to_host(C) == 0.0)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_split_gemm_small_integers_exact(mdg, torch_cuda, method):
+    # exactly representable products and sums: the result must be exact
+    rng = np.random.default_rng(8)
+    L, m, n, k = 4, 33, 17, 65
+    A = np.zeros((m, k, L))
+    B = np.zeros((k, n, L))
+    A[:, :, 0] = rng.integers(-100, 101, (m, k))
+    B[:, :, 0] = rng.integers(-100, 101, (k, n))
+    C, _ = mdg.gemm(L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method=method)
+    C = to_host(C).reshape(m, n, L)
+    assert np.array_equal(C[:, :, 0], A[:, :, 0] @ B[:, :, 0])
+    assert np.all(C[:, :, 1:] == 0.0)
+
+
+def test_split_gemm_rejects_non_finite(mdg, torch_cuda):
+    L, m, n, k = 2, 4, 4, 4
+    A = np.ones(m * k * L)
+    A[5] = np.nan
+    with pytest.raises(mdg.MdgError) as ei:
+        mdg.gemm(L, m, n, k, to_dev(A), to_dev(np.ones(k * n * L)), method="int8")
+    assert ei.value.status == 5
+
+
+def test_int8_widened_accumulation(mdg, orc, torch_cuda):
+    # S * kp > 2^17: the INT8 kernel must widen INT32 sums to INT64 in-loop
+    L, m, n, k = 4, 64, 64, 4608
+    A = orc.generate(orc.GEN_QD, L, m, k, 5, -8, 8)
+    B = orc.generate(orc.GEN_QD, L, k, n, 6, -8, 8)
+    C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method="int8")
+    assert info["flushed"] == 1, info
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=512)
+    assert err <= _bound(L, k), (err, info)

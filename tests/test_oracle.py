@@ -1,0 +1,159 @@
+"""The CPU oracle, pinned: against the reference's own code (golden vectors
+from tests/golden/gen_golden.py, and the reference-built oracle/_ref where it
+exists), against exact rational arithmetic, and against closed forms."""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import assert_bits_equal
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
+LS = (2, 4, 8, 16)
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLDEN, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ------------------------------------------------ golden vectors (reference) --
+@pytest.mark.parametrize("L", LS)
+def test_ops_match_reference_vectors(orc, gold, L):
+    a, b, y = gold[f"L{L}_a"], gold[f"L{L}_b"], gold[f"L{L}_y"]
+    assert_bits_equal(orc.op_mul(L, a, b), gold[f"L{L}_mul"], f"mul L={L}")
+    assert_bits_equal(orc.op_add(L, a, b), gold[f"L{L}_add"], f"add L={L}")
+    assert_bits_equal(orc.op_inc_d(L, a, y), gold[f"L{L}_inc_d"], f"inc_d L={L}")
+    if L > 2:
+        f = gold[f"L{L}_f"]
+        assert_bits_equal(orc.op_fast_renorm(L, f), gold[f"L{L}_fast_renorm"], "fast_renorm")
+        assert_bits_equal(orc.op_renorm(L, f), gold[f"L{L}_renorm"], "renorm")
+
+
+@pytest.mark.parametrize("L", LS)
+@pytest.mark.parametrize("tag", ["qd", "ref"])
+def test_gemm_matches_reference_vectors(orc, gold, L, tag):
+    A, B = gold[f"L{L}_gemm_{tag}_A"], gold[f"L{L}_gemm_{tag}_B"]
+    C = orc.gemm(L, 5, 7, 9, A, B)
+    assert_bits_equal(C, gold[f"L{L}_gemm_{tag}_C"], f"gemm L={L} {tag}")
+    # the fma two_prod agrees with Dekker's away from underflow
+    assert_bits_equal(orc.gemm(L, 5, 7, 9, A, B, mode=orc.TP_FMA), C, "fma gemm")
+
+
+@pytest.mark.parametrize("L", LS)
+def test_split_and_squeeze_match_reference_vectors(orc, gold, L):
+    for tag in ("ref", "qd"):
+        x = gold[f"L{L}_split_{tag}_x"]
+        assert_bits_equal(orc.split4pd(x, L), gold[f"L{L}_split4pd_{tag}"], f"split4pd {tag}")
+        assert_bits_equal(orc.split8pd(x, L), gold[f"L{L}_split8pd_{tag}"], f"split8pd {tag}")
+    assert_bits_equal(orc.squeeze(L, 4 * L, gold[f"L{L}_squeeze_x"]), gold[f"L{L}_squeeze_y"],
+                      "squeeze")
+
+
+def test_mixsplit2_matches_reference_vectors(orc, gold):
+    for tag in ("ref", "qd"):
+        assert_bits_equal(orc.mixsplit2(gold[f"mix_{tag}_x"]), gold[f"mix_{tag}"], tag)
+
+
+# --------------------------------------- reference-built library (fresh data) --
+def _ref_or_skip(orc):
+    if orc.ref() is None and not orc.build_ref():
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+
+
+@pytest.mark.parametrize("L", LS)
+def test_oracle_matches_reference_build_fresh(orc, L):
+    _ref_or_skip(orc)
+    rng = np.random.default_rng(L)
+    a = orc.generate(orc.GEN_QD, L, 3000, 1, 11 * L, -30, 30)
+    b = orc.generate(orc.GEN_QD, L, 3000, 1, 13 * L, -30, 30)
+    a[rng.random(a.size) < 0.1] = 0.0
+    assert_bits_equal(orc.op_mul(L, a, b), orc.op_mul(L, a, b, use_ref=True), "mul")
+    assert_bits_equal(orc.op_add(L, a, b), orc.op_add(L, a, b, use_ref=True), "add")
+    n = 10
+    A = orc.generate(orc.GEN_QD, L, n, n, 3, -4, 4)
+    B = orc.generate(orc.GEN_QD, L, n, n, 4, -4, 4)
+    assert_bits_equal(orc.gemm(L, n, n, n, A, B), orc.gemm(L, n, n, n, A, B, use_ref=True), "gemm")
+
+
+# ------------------------------------------------- exact checker, pinned ------
+def _frac_entry(L, k, A, B, n, r, c):
+    s = Fraction(0)
+    for kk in range(k):
+        a = sum(Fraction(float(v)) for v in A[(r * k + kk) * L:(r * k + kk + 1) * L])
+        b = sum(Fraction(float(v)) for v in B[(kk * n + c) * L:(kk * n + c + 1) * L])
+        s += a * b
+    return s
+
+
+@pytest.mark.parametrize("L", LS)
+def test_exact_checker_equals_rational_arithmetic(orc, L):
+    m, n, k = 3, 4, 6
+    A = orc.generate(orc.GEN_QD, L, m, k, 21, -40, 40)
+    B = orc.generate(orc.GEN_QD, L, k, n, 22, -40, 40)
+    for r in range(m):
+        for c in range(n):
+            v, base = orc.exact_entry(L, k, A, k, B, n, r, c)
+            assert Fraction(v) * Fraction(2) ** base == _frac_entry(L, k, A, B, n, r, c)
+
+
+def test_exact_checker_error_of_exact_result_is_zero(orc):
+    # small integers: every product and sum is exact in one double
+    rng = np.random.default_rng(5)
+    m, n, k, L = 4, 5, 6, 4
+    A = np.zeros(m * k * L)
+    B = np.zeros(k * n * L)
+    A[::L] = rng.integers(-9, 10, m * k)
+    B[::L] = rng.integers(-9, 10, k * n)
+    Cint = (A[::L].reshape(m, k) @ B[::L].reshape(k, n)).ravel()
+    C = np.zeros(m * n * L)
+    C[::L] = Cint
+    rr, cc = np.divmod(np.arange(m * n), n)
+    err, den, val = orc.exact_error(L, m, n, k, A, B, C, rr, cc)
+    assert np.all(err == 0.0) and np.array_equal(val, Cint)
+    # a planted mistake in one limb shows up exactly
+    C2 = C.copy()
+    C2[(2 * n + 3) * L + 1] = 2.0 ** -60
+    err2, _, _ = orc.exact_error(L, m, n, k, A, B, C2, rr, cc)
+    assert err2[2 * n + 3] == -(2.0 ** -60) and np.count_nonzero(err2) == 1
+
+
+# ------------------------------------------------------------ closed forms --
+@pytest.mark.parametrize("L", LS)
+def test_gemm_identity_is_exact(orc, L):
+    # A*I has exactly A's value (the reference generator's limbs overlap, so
+    # the first pass renormalises them), and a renormalised A is a fixed point
+    n = 6
+    A = orc.generate(orc.GEN_REF, L, n, n, 31, 0, 0)
+    ident = np.zeros(n * n * L)
+    ident[(np.arange(n) * n + np.arange(n)) * L] = 1.0
+    AI = orc.gemm(L, n, n, n, A, ident)
+    for e in range(n * n):
+        assert (sum(Fraction(float(v)) for v in AI[e * L:(e + 1) * L])
+                == sum(Fraction(float(v)) for v in A[e * L:(e + 1) * L]))
+    assert_bits_equal(orc.gemm(L, n, n, n, AI, ident), AI, "(A*I)*I")
+    assert_bits_equal(orc.gemm(L, n, n, n, ident, AI), AI, "I*(A*I)")
+
+
+@pytest.mark.parametrize("L", LS)
+def test_gemm_error_within_bound(orc, L):
+    m, n, k = 6, 5, 40
+    A = orc.generate(orc.GEN_QD, L, m, k, 41, -8, 8)
+    B = orc.generate(orc.GEN_QD, L, k, n, 42, -8, 8)
+    C = orc.gemm(L, m, n, k, A, B)
+    err, _ = orc.normwise_error(L, m, n, k, A, B, C)
+    assert err <= 8 * k * 2.0 ** (-53 * L + (2 if L == 2 else 6))
+
+
+def test_split_parts_sum_to_stored_value(orc):
+    # positive inputs of the reference generator: the parts add back exactly
+    for L in (2, 4):
+        x = orc.generate(orc.GEN_REF, L, 50, 1, 51, -3, 3)
+        for parts, per in ((orc.split4pd(x, L), 4), (orc.split8pd(x, L), 8)):
+            per_entry = parts.reshape(-1, per * L)
+            for i, v in enumerate(x.reshape(-1, L)):
+                want = sum(Fraction(float(t)) for t in v)
+                got = sum(Fraction(float(t)) for t in per_entry[i])
+                assert got == want
