@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Effective quad-double GEMM throughput at n = 4096 on one MI355X.
+
+One step = one C = A*B of 4096 x 4096 quad-doubles (4 FP64 limbs, leading
+limb s*2^e with e uniform in [-8, 8], s ~ +-U[1,2), each next limb the
+previous * 2^-53 * U[-0.5, 0.5)), inputs resident in HBM, through the
+INT8-slice path.  The FP64-MFMA split path, the BF16-slice path and the plain
+vector quad-double GEMM run on the same inputs; every path's normwise error
+against the exact checker is reported beside its throughput.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  Effective GFLOP/s = 2 m n k / time.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = "multiple-double-arithmetic-matrix-matrix-multiplication_amd"
+sys.path.insert(0, ROOT)
+
+INT8_PEAK_TOPS = 5000.0     # MI355X dense INT8 MFMA, tera-ops/s
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 matrix / vector
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--L", type=int, default=4)
+    ap.add_argument("--samples", type=int, default=1024, help="entries checked exactly per path")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--skip-vector", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--paths", default="int8,f64,bf16,vector")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
+              "sharding is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    mdg = importlib.import_module(PKG)
+    torch.cuda.set_device(0)
+    L, n = a.L, a.n
+    m = k = n
+    eff_flops = 2.0 * m * n * k
+    seeds = (1, 2)
+    A = mdg.fill_random(mdg.GEN_QD, L, m, k, seeds[0], -8, 8)
+    B = mdg.fill_random(mdg.GEN_QD, L, k, n, seeds[1], -8, 8)
+    C = mdg.empty(m * n * L)
+    torch.cuda.synchronize()
+    paths = [p for p in a.paths.split(",") if p]
+    if a.skip_vector and "vector" in paths:
+        paths.remove("vector")
+    results = {}
+    outputs = {}
+    mdg.set_profiling(True)
+    for path in paths:
+        steps = a.steps if path == "int8" else (1 if path == "vector" else max(1, a.steps // 4))
+        warm = a.warmup if path != "vector" else 0
+        if path == "vector":
+            # load the kernel on a small problem first
+            s = mdg.fill_random(mdg.GEN_QD, L, 64, 64, 3, -8, 8)
+            mdg.gemm(L, 64, 64, 64, s, s, method="vector")
+        log(f"{path}: warmup {warm}, steps {steps}")
+        for _ in range(warm):
+            mdg.gemm(L, m, n, k, A, B, method=path, C=C)
+        torch.cuda.synchronize()
+        stage = np.zeros(4)
+        info = None
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _, info = mdg.gemm(L, m, n, k, A, B, method=path, C=C)
+            stage += np.array(info["stage_ms"])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        stage /= steps
+        outputs[path] = C.cpu().numpy()
+        results[path] = {"ms": ms, "gflops_eff": eff_flops / ms / 1e6, "stage_ms": stage.tolist(),
+                         "slices": info["slices"], "slice_bits": info["slice_bits"],
+                         "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
+                         "spread": [info["spread_a"], info["spread_b"]], "steps": steps}
+        log(f"{path}: {ms:.2f} ms/step, {eff_flops / ms / 1e6:.1f} GFLOP/s effective, "
+            f"stages {np.round(stage, 3).tolist()} ms")
+    mdg.set_profiling(False)
+
+    # exact normwise error of every path on sampled entries
+    import pyoracle as orc  # noqa: E402  (the checker)
+    orc.build()
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    for path, Ch in outputs.items():
+        t0 = time.perf_counter()
+        err, ns = orc.normwise_error(L, m, n, k, Ah, Bh, Ch, nsamp=a.samples, seed=7)
+        results[path]["normwise_error"] = err
+        results[path]["error_bound"] = 4.0 * k * 2.0 ** (-53 * L)
+        log(f"{path}: normwise error {err:.3e} on {ns} entries ({time.perf_counter() - t0:.1f} s)")
+
+    head = results["int8"] if "int8" in results else next(iter(results.values()))
+    head_path = "int8" if "int8" in results else next(iter(results))
+    P = head["pair_gemms"]
+    gemm_ms = head["stage_ms"][2]
+    if head_path == "int8":
+        work = P * 2.0 * m * n * k
+        peak, unit = INT8_PEAK_TOPS, "TFLOP/s"
+        kernel = "diag_i8_kernel"
+    elif head_path == "bf16":
+        work = P * 2.0 * m * n * k
+        peak, unit = BF16_PEAK_TFLOPS, "TFLOP/s"
+        kernel = "diag_bf16_kernel"
+    else:
+        work = P * 2.0 * m * n * k
+        peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
+        kernel = "diag_f64_kernel"
+    achieved = work / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    workload = f"qd{n}"
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": (achieved / peak) if achieved else None,
+                "traffic": read_traffic(kernel, workload), "kernel": kernel,
+                "work_per_launch": work, "avg_launch_ms": gemm_ms}
+
+    cpu = None
+    if not a.skip_cpu:
+        nthreads = orc.num_threads()
+        # time rows of the reference's host GEMM restatement until the budget is spent
+        r0, rows, t_cpu = 0, 0, 0.0
+        chunk = max(1, nthreads)
+        while t_cpu < a.cpu_seconds and rows < m:
+            t0 = time.perf_counter()
+            orc.gemm_rows(L, r0, min(m, r0 + chunk), n, k, Ah, Bh, mode=orc.TP_DEKKER,
+                          nthreads=nthreads)
+            t_cpu += time.perf_counter() - t0
+            rows += min(m, r0 + chunk) - r0
+            r0 += chunk
+        cpu_gflops = 2.0 * rows * n * k / t_cpu / 1e9
+        cpu = {"value": cpu_gflops, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+               "sample": f"{rows} of {m} output rows of the qd n={n} GEMM (host loops of "
+                         f"C++/TCcuda.cu:334-346 restated, Dekker two_prod), {t_cpu:.1f} s, "
+                         f"scaled by effective 2*rows*n*k flops"}
+        log(f"cpu baseline: {cpu_gflops:.3f} GFLOP/s on {nthreads} threads ({rows} rows)")
+
+    line = {
+        "metric": "effective qd-GEMM GFLOP/s at n=4096: INT8-slice vs FP64-MFMA split vs vector, same err",
+        "value": head["gflops_eff"],
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": head["steps"],
+        "warmup": a.warmup,
+        "ms_per_step": head["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8" if head_path == "int8" else ("bf16" if head_path == "bf16" else "f64"),
+        "data": "synthetic",
+        "config": {"workload": f"quad-double GEMM C=A*B, m=n=k={n}, 4 fp64 limbs, leading exponent "
+                               f"e in [-8,8], seeds A={seeds[0]} B={seeds[1]}",
+                   "L": L, "m": m, "n": n, "k": k, "path": head_path},
+        "paths": results,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

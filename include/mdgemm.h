@@ -60,6 +60,10 @@ typedef struct mdg_gemm_info {
     int spread_a;          /* exponent spread estimate of A, bits */
     int spread_b;          /* exponent spread estimate of B, bits */
     int flushed;           /* 1 if accumulators were widened inside the K loop */
+    /* stage times in ms, measured with HIP events on the call's stream when
+       profiling is on (mdg_set_profiling), else 0:
+       [0] exponent scan, [1] slicing, [2] diagonal GEMMs, [3] reconstruction */
+    float stage_ms[4];
 } mdg_gemm_info;
 
 /* One problem of a batch (all pointers on the device). */
@@ -82,6 +86,10 @@ int mdg_init(void);
 
 /* Release cached scratch buffers of the current device. */
 int mdg_release_workspace(void);
+
+/* Per-stage HIP-event timing of mdg_gemm (on = 1) into mdg_gemm_info.stage_ms;
+   when on, mdg_gemm waits for its own last event before returning. */
+int mdg_set_profiling(int on);
 
 /* ----------------------------------------------------------------------
  * Vector GEMM.  C = A * B with one thread per output entry, each a k-step

@@ -1,5 +1,6 @@
 // api.hip -- the extern "C" entry points of libmdgemm.so (include/mdgemm.h):
 // argument validation, status codes, scratch management and dispatch.
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -25,6 +26,9 @@ int batched_gemm(int count, const mdg_problem *problems, int method, int target_
                  int persistent, hipStream_t s);
 
 static thread_local char g_err[512] = "";
+static std::atomic<int> g_profiling{0};
+
+bool profiling_on() { return g_profiling.load(std::memory_order_relaxed) != 0; }
 
 void set_error(const char *fmt, ...)
 {
@@ -108,7 +112,7 @@ static int check_gemm_args(int L, int m, int n, int k, const void *A, long long 
 {
     MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d (expected 2, 4, 8 or 16)", L);
     MDG_REQUIRE(m >= 0 && n >= 0 && k >= 0, MDG_ERROR_INVALID_VALUE, "negative dimension (m=%d n=%d k=%d)", m, n, k);
-    MDG_REQUIRE(lda >= (k > 0 ? k : 1) && ldb >= (n > 0 ? n : 1) && ldc >= (n > 0 ? n : 1), MDG_ERROR_INVALID_VALUE,
+    MDG_REQUIRE(lda >= k && ldb >= n && ldc >= n, MDG_ERROR_INVALID_VALUE,
                 "leading dimension too small (lda=%lld >= k=%d, ldb=%lld >= n=%d, ldc=%lld >= n=%d)", lda, k, ldb, n,
                 ldc, n);
     MDG_REQUIRE((m == 0 || n == 0) || C, MDG_ERROR_INVALID_VALUE, "C is NULL");
@@ -154,6 +158,12 @@ int mdg_init(void)
 
 int mdg_release_workspace(void) { return release_scratch(); }
 
+int mdg_set_profiling(int on)
+{
+    g_profiling.store(on ? 1 : 0);
+    return MDG_SUCCESS;
+}
+
 int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
                     double *C, long long ldc, int twoprod, void *stream)
 {
@@ -181,8 +191,22 @@ int mdg_gemm(int L, int m, int n, int k, const double *A, long long lda, const d
     }
     if (m == 0 || n == 0) return MDG_SUCCESS;
     if (k == 0) return zero_c(C, m, n, ldc, L, as_stream(stream));
-    if (method == MDG_METHOD_VECTOR)
-        return vector_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, MDG_TWOPROD_FMA, as_stream(stream));
+    if (method == MDG_METHOD_VECTOR) {
+        StageTimer tm(as_stream(stream), profiling_on() && info != nullptr);
+        tm.mark(0);
+        tm.mark(1);
+        tm.mark(2);
+        rc = vector_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, MDG_TWOPROD_FMA, as_stream(stream));
+        if (rc) return rc;
+        tm.mark(3);
+        tm.mark(4);
+        if (tm.on) {
+            float st[4];
+            MDG_HIP_CHECK(tm.elapsed(st));
+            for (int i = 0; i < 4; ++i) info->stage_ms[i] = st[i];
+        }
+        return MDG_SUCCESS;
+    }
     return split_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, -1, info, as_stream(stream));
 }
 

@@ -56,6 +56,38 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // buffers, so concurrent streams never alias scratch.
 void *scratch(void *stream, size_t bytes, int slot);
 
+// mdg_set_profiling state
+bool profiling_on();
+
+// HIP events at stage boundaries of one call, on the call's stream
+struct StageTimer {
+    bool on;
+    hipStream_t s;
+    hipEvent_t ev[5] = {};
+    StageTimer(hipStream_t stream, bool enabled) : on(enabled), s(stream)
+    {
+        if (on)
+            for (auto &e : ev)
+                if (hipEventCreate(&e) != hipSuccess) on = false;
+    }
+    void mark(int i)
+    {
+        if (on) (void)hipEventRecord(ev[i], s);
+    }
+    // elapsed ms between consecutive marks 0..4 (waits for the last mark)
+    hipError_t elapsed(float *ms)
+    {
+        hipError_t e = hipEventSynchronize(ev[4]);
+        for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        return e;
+    }
+    ~StageTimer()
+    {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // counter-based generator used by mdg_fill_random and the split kernels' tests
 __host__ __device__ inline uint64_t mix64(uint64_t z)
 {

@@ -292,15 +292,30 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
             }
         }
 
+    // the next k-block is fetched into registers while the current one is
+    // multiplied out of LDS
+    uint4 ra[4], rb[4];
+    auto fetch = [&](int kb) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            const long long off = (long long)(idx >> 3) * ld + kb + (idx & 7) * 16;
+            ra[it] = *(const uint4 *)(Ab + off);
+            rb[it] = *(const uint4 *)(Bb + off);
+        }
+    };
+    fetch(0);
     int blk = 0;
     for (int kb = 0; kb < Kd; kb += BK, ++blk) {
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             const int idx = tid + it * 256;
-            const int row = idx >> 3, seg = (idx & 7) * 16;
-            *(uint4 *)&As[row * LDR + seg] = *(const uint4 *)(Ab + (long long)row * ld + kb + seg);
-            *(uint4 *)&Bs[row * LDR + seg] = *(const uint4 *)(Bb + (long long)row * ld + kb + seg);
+            const int o = (idx >> 3) * LDR + (idx & 7) * 16;
+            *(uint4 *)&As[o] = ra[it];
+            *(uint4 *)&Bs[o] = rb[it];
         }
+        __syncthreads();
+        if (kb + BK < Kd) fetch(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 64) {
@@ -352,15 +367,16 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
 }
 
 // --------------------------------------------------- BF16 diagonal GEMM --
-// Same tiling as INT8 with 64 bf16 (128 bytes) per k-block row and the
-// 16x16x32 bf16 MFMA; FP32 sums stay exact below 2^24, so every 16 k-blocks
-// (1024 products of at most 2^14) they are moved into INT64.
+// 64 bf16 (128 bytes) per k-block row and the 16x16x32 bf16 MFMA.  FP32 sums
+// stay exact below 2^24, so every 16 k-blocks (1024 products of at most
+// 2^14) they are moved into INT64 registers; to leave room for those the
+// workgroup tile is 128 x 64 (waves 2 x 2, each 64 x 32).
 __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict__ Aall,
                                                         const __bf16 *__restrict__ Ball, int S, int kp,
                                                         int m, int n, int tiles_m, int tiles_n,
                                                         long long *__restrict__ Dout)
 {
-    constexpr int BM = 128, BN = 128, BK = 64, LDR = BK + 8;   // elements
+    constexpr int BM = 128, BN = 64, BK = 64, LDR = BK + 8;   // elements
     __shared__ __attribute__((aligned(16))) __bf16 As[BM * LDR];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * LDR];
 
@@ -376,40 +392,59 @@ __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict
     const int wm = wave >> 1, wn = wave & 1;
     const int lrow = lane & 15, lk = (lane >> 4) * 8;
 
-    f4 acc[4][4];
-    long long wide[4][4][4];
+    f4 acc[4][2];
+    long long wide[4][2][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
         }
 
+    uint4 ra[4], rb[2];
+    auto fetch = [&](int kb) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            ra[it] = *(const uint4 *)(Ab + (long long)(idx >> 3) * ld + kb + (idx & 7) * 8);
+        }
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int idx = tid + it * 256;
+            rb[it] = *(const uint4 *)(Bb + (long long)(idx >> 3) * ld + kb + (idx & 7) * 8);
+        }
+    };
+    fetch(0);
     int blk = 0;
     for (int kb = 0; kb < Kd; kb += BK, ++blk) {
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             const int idx = tid + it * 256;
-            const int row = idx >> 3, seg = (idx & 7) * 8;
-            *(uint4 *)&As[row * LDR + seg] = *(const uint4 *)(Ab + (long long)row * ld + kb + seg);
-            *(uint4 *)&Bs[row * LDR + seg] = *(const uint4 *)(Bb + (long long)row * ld + kb + seg);
+            *(uint4 *)&As[(idx >> 3) * LDR + (idx & 7) * 8] = ra[it];
         }
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int idx = tid + it * 256;
+            *(uint4 *)&Bs[(idx >> 3) * LDR + (idx & 7) * 8] = rb[it];
+        }
+        __syncthreads();
+        if (kb + BK < Kd) fetch(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
-            bf8 a[4], b[4];
+            bf8 a[4], b[2];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 a[i] = *(const bf8 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                b[j] = *(const bf8 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
+            for (int j = 0; j < 2; ++j)
+                b[j] = *(const bf8 *)&Bs[(wn * 32 + j * 16 + lrow) * LDR + ks + lk];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
@@ -417,7 +452,7 @@ __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 2; ++j) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) wide[i][j][v] += (long long)acc[i][j][v];
                     acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
@@ -428,8 +463,8 @@ __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long col = (long long)tn * BN + wn * 64 + j * 16 + (lane & 15);
+        for (int j = 0; j < 2; ++j) {
+            const long long col = (long long)tn * BN + wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long row = (long long)tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
@@ -480,25 +515,38 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
             for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
         }
 
-    int blk = 0;
-    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
-        // A: 128 rows x 16 k = 2048 values, 8 per thread; B: 64 x 16 = 1024, 4 per thread
+    // A: 128 rows x 16 k = 2048 values, 8 per thread; B: 64 x 16 = 1024, 4
+    // per thread; fetched one k-block ahead into registers
+    TA ra[8], rb[4];   // raw slices; converted to double on the way into LDS
+    auto fetch = [&](int kb) {
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int idx = tid + it * 256;
             const int row = idx >> 4, kk = idx & 15;
-            double v = 0.0;
-            if (row0 + row < m && kb + kk < Kd) v = (double)Ab[(long long)row * lda + kb + kk];
-            As[row * LDR + kk] = v;
+            ra[it] = (row0 + row < m && kb + kk < Kd) ? Ab[(long long)row * lda + kb + kk] : (TA)0;
         }
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             const int idx = tid + it * 256;
             const int row = idx >> 4, kk = idx & 15;
-            double v = 0.0;
-            if (col0 + row < n && kb + kk < Kd) v = (double)Bb[(long long)row * ldb + kb + kk];
-            Bs[row * LDR + kk] = v;
+            rb[it] = (col0 + row < n && kb + kk < Kd) ? Bb[(long long)row * ldb + kb + kk] : (TA)0;
         }
+    };
+    fetch(0);
+    int blk = 0;
+    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int idx = tid + it * 256;
+            As[(idx >> 4) * LDR + (idx & 15)] = (double)ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + it * 256;
+            Bs[(idx >> 4) * LDR + (idx & 15)] = (double)rb[it];
+        }
+        __syncthreads();
+        if (kb + BK < Kd) fetch(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
@@ -535,7 +583,8 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
             const long long col = col0 + wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const long long row = row0 + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                // v_mfma_f64_16x16x4f64: element v of lane l is row l/16 + 4v, col l%16
+                const long long row = row0 + wm * 64 + i * 16 + (lane >> 4) + 4 * v;
                 if (row < m && col < n) {
                     const long long o = plane + row * ldd + col;
                     if constexpr (OUT == 0)
@@ -653,6 +702,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     double *colsum = (double *)scratch(s, sizeof(double) * (size_t)n, 1);
     MDG_REQUIRE(colsum, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
 
+    StageTimer tm(s, profiling_on());
+    tm.mark(0);
     hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, F, (long long)n, INT_MIN);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, stats, 8ll, 0);
     hipLaunchKernelGGL(fill_double_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, colsum, (long long)n, 0.0);
@@ -662,6 +713,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                        ldb, rpb, F, colsum, stats);
     hipLaunchKernelGGL(finish_cols_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, k, n, F, colsum, stats);
     MDG_LAUNCH_CHECK("scan kernels");
+    tm.mark(1);
     if (fixed_spread < 0) {
         int hstats[4];
         MDG_HIP_CHECK(hipMemcpyAsync(hstats, stats, sizeof(hstats), hipMemcpyDeviceToHost, s));
@@ -704,6 +756,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                            s, m, k, mp, kp, A, lda, E, S, w, Aall);
         hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        tm.mark(2);
         const int tiles_n = np / 128;
         dim3 grid(tiles_m * tiles_n, 1, zdim);
         if (p.wide)
@@ -713,6 +766,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             hipLaunchKernelGGL((diag_i8_kernel<false>), grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m,
                                tiles_n, p.flush_blocks, D);
         MDG_LAUNCH_CHECK("diag_i8_kernel");
+        tm.mark(3);
         if (p.wide)
             hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m,
                                n, S, w, (const long long *)D, E, F, C, ldc);
@@ -730,10 +784,12 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                            s, m, k, mp, kp, A, lda, E, S, w, Aall);
         hipLaunchKernelGGL((slice_b_kernel<L, __bf16>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
-        const int tiles_n = np / 128;
+        tm.mark(2);
+        const int tiles_n = np / 64;
         dim3 grid(tiles_m * tiles_n, 1, zdim);
         hipLaunchKernelGGL(diag_bf16_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m, tiles_n, D);
         MDG_LAUNCH_CHECK("diag_bf16_kernel");
+        tm.mark(3);
         hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
                            S, w, (const long long *)D, E, F, C, ldc);
     } else {
@@ -746,16 +802,21 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                            s, m, k, mp, kp, A, lda, E, S, w, Aall);
         hipLaunchKernelGGL((slice_b_kernel<L, int32_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        tm.mark(2);
         const int tiles_n = np / 64;
         dim3 grid(tiles_m * tiles_n, 1, zdim);
         hipLaunchKernelGGL((diag_f64_kernel<int32_t, 2>), grid, dim3(256), 0, s, Aall, Ball, S, kp,
                            (long long)S * kp, (long long)S * kp, m, n, tiles_m, tiles_n, p.flush_blocks, (void *)D,
                            (long long)n);
         MDG_LAUNCH_CHECK("diag_f64_kernel");
+        tm.mark(3);
         hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
                            S, w, (const long long *)D, E, F, C, ldc);
     }
     MDG_LAUNCH_CHECK("split gemm");
+    tm.mark(4);
+    float stage[4] = {0.f, 0.f, 0.f, 0.f};
+    if (tm.on) MDG_HIP_CHECK(tm.elapsed(stage));
     if (info) {
         info->method = p.method;
         info->slices = p.S;
@@ -766,6 +827,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         info->spread_a = p.spread_a;
         info->spread_b = p.spread_b;
         info->flushed = p.wide ? 1 : 0;
+        for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;
 }

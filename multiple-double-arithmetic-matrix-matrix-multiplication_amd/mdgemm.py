@@ -31,10 +31,13 @@ class GemmInfo(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("slices", ctypes.c_int), ("slice_bits", ctypes.c_int),
                 ("diagonals", ctypes.c_int), ("pair_gemms", ctypes.c_longlong),
                 ("total_bits", ctypes.c_int), ("spread_a", ctypes.c_int),
-                ("spread_b", ctypes.c_int), ("flushed", ctypes.c_int)]
+                ("spread_b", ctypes.c_int), ("flushed", ctypes.c_int),
+                ("stage_ms", ctypes.c_float * 4)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["stage_ms"] = list(self.stage_ms)
+        return d
 
 
 class Problem(ctypes.Structure):
@@ -52,6 +55,7 @@ _SIGS = {
     "mdg_version": ([], _i),
     "mdg_init": ([], _i),
     "mdg_release_workspace": ([], _i),
+    "mdg_set_profiling": ([_i], _i),
     "mdg_vector_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _vp], _i),
     "mdg_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _i, ctypes.POINTER(GemmInfo), _vp], _i),
     "mdg_gemm_batched": ([_i, ctypes.POINTER(Problem), _i, _i, _i, _i, _vp], _i),
@@ -106,6 +110,10 @@ def _stream(stream):
 
 
 # --------------------------------------------------------------- wrappers --
+def set_profiling(on):
+    _check(lib().mdg_set_profiling(1 if on else 0))
+
+
 def empty(n, device="cuda"):
     import torch
     return torch.empty(int(n), dtype=torch.float64, device=device)

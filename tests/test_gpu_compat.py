@@ -1,0 +1,47 @@
+"""The reference's C++ interface (include/mdgemm_compat.hpp) end to end, driven
+by tests/cpp/tc_compat_test.cpp the way C++/TCtest.cu drives the reference,
+checked against the oracle bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(ROOT, "tests", "cpp", "tc_compat_test")
+
+
+@pytest.mark.parametrize("p", [2, 4, 8, 16])
+def test_compat_driver(orc, tmp_path, p):
+    assert os.path.exists(EXE), "tests/cpp/tc_compat_test is not built; run __graft_entry__.build()"
+    n = 16
+    pp = 4 * p
+    r = subprocess.run([EXE, str(tmp_path), str(n), str(p)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout)
+    ld = {f[:-4]: np.fromfile(tmp_path / f, dtype=np.float64) for f in os.listdir(tmp_path)}
+    A = orc.generate(orc.GEN_REF, p, n, n, 2024, 0, 0)
+    B = orc.generate(orc.GEN_REF, p, n, n, 2025, 0, 0)
+    assert_bits_equal(ld["A"], A, "mat A")
+    assert_bits_equal(ld["B"], B, "mat B")
+    Aq, Bq = orc.split4pd(A, p), orc.split4pd(B, p)
+    assert_bits_equal(ld["Aq"], Aq, "split4pd")
+    assert_bits_equal(ld["Bq"], Bq, "split4pd")
+    assert_bits_equal(ld["B2"], orc.bigB2(n, pp, Bq), "bigB2")
+    C1q = orc.conv_gemm(n, n, n, pp, Aq, Bq)
+    assert_bits_equal(ld["C1q"], C1q, "Aq * bigB2(Bq)")
+    assert_bits_equal(ld["C1"], orc.squeeze(p, pp, C1q), "pllsqueeze_old")
+    assert_bits_equal(ld["C2"], orc.gemm(p, n, n, n, A, B, mode=orc.TP_FMA), "matmulTCnt")
+    assert_bits_equal(ld["C3"], orc.gemm(p, n, n, n, A, B, mode=orc.TP_DEKKER), "matmul*f")
+    assert_bits_equal(ld["part1"], A[1::p], "part")
+    A3 = A.reshape(n, n, p)
+    assert_bits_equal(ld["bigA"], A3.transpose(0, 2, 1).ravel(), "bigA")
+    B3 = B.reshape(n, n, p)
+    want = np.zeros((p, n, p, n))     # [i][c][j][r] column-major blocks
+    for i in range(p):
+        for j in range(i + 1):
+            want[i, :, j, :] = B3[:, :, i - j].T
+    assert_bits_equal(ld["bigB"], want.ravel(), "bigB")
