@@ -22,6 +22,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = "multiple-double-arithmetic-matrix-matrix-multiplication_amd"
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 INT8_PEAK_TOPS = 5000.0     # MI355X dense INT8 MFMA, tera-ops/s
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 matrix / vector

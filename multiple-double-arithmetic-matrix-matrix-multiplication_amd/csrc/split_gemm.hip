@@ -293,29 +293,35 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
         }
 
     // the next k-block is fetched into registers while the current one is
-    // multiplied out of LDS
-    uint4 ra[4], rb[4];
-    auto fetch = [&](int kb) {
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            const long long off = (long long)(idx >> 3) * ld + kb + (idx & 7) * 16;
-            ra[it] = *(const uint4 *)(Ab + off);
-            rb[it] = *(const uint4 *)(Bb + off);
-        }
-    };
-    fetch(0);
+    // multiplied out of LDS: thread tid moves 16 bytes of rows tid/8 + 32 i
+    const int frow = tid >> 3, fseg = (tid & 7) * 16;
+    const long long foff = (long long)frow * ld + fseg, fstep = 32 * ld;
+    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define MDG_FETCH_I8(kb)                                      \
+    do {                                                      \
+        ra0 = *(const uint4 *)(Ab + foff + (kb));             \
+        ra1 = *(const uint4 *)(Ab + foff + fstep + (kb));     \
+        ra2 = *(const uint4 *)(Ab + foff + 2 * fstep + (kb)); \
+        ra3 = *(const uint4 *)(Ab + foff + 3 * fstep + (kb)); \
+        rb0 = *(const uint4 *)(Bb + foff + (kb));             \
+        rb1 = *(const uint4 *)(Bb + foff + fstep + (kb));     \
+        rb2 = *(const uint4 *)(Bb + foff + 2 * fstep + (kb)); \
+        rb3 = *(const uint4 *)(Bb + foff + 3 * fstep + (kb)); \
+    } while (0)
+    MDG_FETCH_I8(0);
     int blk = 0;
     for (int kb = 0; kb < Kd; kb += BK, ++blk) {
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            const int o = (idx >> 3) * LDR + (idx & 7) * 16;
-            *(uint4 *)&As[o] = ra[it];
-            *(uint4 *)&Bs[o] = rb[it];
-        }
+        *(uint4 *)&As[soff] = ra0;
+        *(uint4 *)&As[soff + sstep] = ra1;
+        *(uint4 *)&As[soff + 2 * sstep] = ra2;
+        *(uint4 *)&As[soff + 3 * sstep] = ra3;
+        *(uint4 *)&Bs[soff] = rb0;
+        *(uint4 *)&Bs[soff + sstep] = rb1;
+        *(uint4 *)&Bs[soff + 2 * sstep] = rb2;
+        *(uint4 *)&Bs[soff + 3 * sstep] = rb3;
         __syncthreads();
-        if (kb + BK < Kd) fetch(kb + BK);
+        if (kb + BK < Kd) MDG_FETCH_I8(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 64) {
@@ -403,34 +409,31 @@ __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict
             for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
         }
 
-    uint4 ra[4], rb[2];
-    auto fetch = [&](int kb) {
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            ra[it] = *(const uint4 *)(Ab + (long long)(idx >> 3) * ld + kb + (idx & 7) * 8);
-        }
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const int idx = tid + it * 256;
-            rb[it] = *(const uint4 *)(Bb + (long long)(idx >> 3) * ld + kb + (idx & 7) * 8);
-        }
-    };
-    fetch(0);
+    // fetched one k-block ahead: thread tid moves 8 bf16 of rows tid/8 + 32 i
+    const int frow = tid >> 3, fseg = (tid & 7) * 8;
+    const long long foff = (long long)frow * ld + fseg, fstep = 32 * ld;
+    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1;
+#define MDG_FETCH_BF16(kb)                                    \
+    do {                                                      \
+        ra0 = *(const uint4 *)(Ab + foff + (kb));             \
+        ra1 = *(const uint4 *)(Ab + foff + fstep + (kb));     \
+        ra2 = *(const uint4 *)(Ab + foff + 2 * fstep + (kb)); \
+        ra3 = *(const uint4 *)(Ab + foff + 3 * fstep + (kb)); \
+        rb0 = *(const uint4 *)(Bb + foff + (kb));             \
+        rb1 = *(const uint4 *)(Bb + foff + fstep + (kb));     \
+    } while (0)
+    MDG_FETCH_BF16(0);
     int blk = 0;
     for (int kb = 0; kb < Kd; kb += BK, ++blk) {
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            *(uint4 *)&As[(idx >> 3) * LDR + (idx & 7) * 8] = ra[it];
-        }
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const int idx = tid + it * 256;
-            *(uint4 *)&Bs[(idx >> 3) * LDR + (idx & 7) * 8] = rb[it];
-        }
+        *(uint4 *)&As[soff] = ra0;
+        *(uint4 *)&As[soff + sstep] = ra1;
+        *(uint4 *)&As[soff + 2 * sstep] = ra2;
+        *(uint4 *)&As[soff + 3 * sstep] = ra3;
+        *(uint4 *)&Bs[soff] = rb0;
+        *(uint4 *)&Bs[soff + sstep] = rb1;
         __syncthreads();
-        if (kb + BK < Kd) fetch(kb + BK);
+        if (kb + BK < Kd) MDG_FETCH_BF16(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
@@ -518,21 +521,22 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
     // A: 128 rows x 16 k = 2048 values, 8 per thread; B: 64 x 16 = 1024, 4
     // per thread; fetched one k-block ahead into registers
     TA ra[8], rb[4];   // raw slices; converted to double on the way into LDS
-    auto fetch = [&](int kb) {
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int idx = tid + it * 256;
-            const int row = idx >> 4, kk = idx & 15;
-            ra[it] = (row0 + row < m && kb + kk < Kd) ? Ab[(long long)row * lda + kb + kk] : (TA)0;
-        }
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            const int row = idx >> 4, kk = idx & 15;
-            rb[it] = (col0 + row < n && kb + kk < Kd) ? Bb[(long long)row * ldb + kb + kk] : (TA)0;
-        }
-    };
-    fetch(0);
+#define MDG_FETCH3(kb) \
+    do { \
+_Pragma("unroll") \
+        for (int it = 0; it < 8; ++it) { \
+            const int idx = tid + it * 256; \
+            const int row = idx >> 4, kk = idx & 15; \
+            ra[it] = (row0 + row < m && kb + kk < Kd) ? Ab[(long long)row * lda + kb + kk] : (TA)0; \
+        } \
+_Pragma("unroll") \
+        for (int it = 0; it < 4; ++it) { \
+            const int idx = tid + it * 256; \
+            const int row = idx >> 4, kk = idx & 15; \
+            rb[it] = (col0 + row < n && kb + kk < Kd) ? Bb[(long long)row * ldb + kb + kk] : (TA)0; \
+        } \
+    } while (0)
+    MDG_FETCH3(0);
     int blk = 0;
     for (int kb = 0; kb < Kd; kb += BK, ++blk) {
 #pragma unroll
@@ -546,7 +550,7 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
             Bs[(idx >> 4) * LDR + (idx & 15)] = (double)rb[it];
         }
         __syncthreads();
-        if (kb + BK < Kd) fetch(kb + BK);
+        if (kb + BK < Kd) MDG_FETCH3(kb + BK);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
