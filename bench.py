@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--paths", default="int8,f64,bf16,vector")
+    ap.add_argument("--paths", default="int8,int8_mfma,f64,bf16,vector",
+                    help="int8 = INT8 slices with rocBLAS diagonal GEMMs, int8_mfma = with this "
+                         "library's INT8 MFMA kernel")
     return ap.parse_args()
 
 
@@ -92,14 +94,16 @@ def main():
             s = mdg.fill_random(mdg.GEN_QD, L, 64, 64, 3, -8, 8)
             mdg.gemm(L, 64, 64, 64, s, s, method="vector")
         log(f"{path}: warmup {warm}, steps {steps}")
+        method = "int8" if path == "int8_mfma" else path
+        mdg.set_int8_backend(2 if path == "int8_mfma" else 0)
         for _ in range(warm):
-            mdg.gemm(L, m, n, k, A, B, method=path, C=C)
+            mdg.gemm(L, m, n, k, A, B, method=method, C=C)
         torch.cuda.synchronize()
         stage = np.zeros(4)
         info = None
         t0 = time.perf_counter()
         for _ in range(steps):
-            _, info = mdg.gemm(L, m, n, k, A, B, method=path, C=C)
+            _, info = mdg.gemm(L, m, n, k, A, B, method=method, C=C)
             stage += np.array(info["stage_ms"])
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
@@ -108,10 +112,12 @@ def main():
         results[path] = {"ms": ms, "gflops_eff": eff_flops / ms / 1e6, "stage_ms": stage.tolist(),
                          "slices": info["slices"], "slice_bits": info["slice_bits"],
                          "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
-                         "spread": [info["spread_a"], info["spread_b"]], "steps": steps}
+                         "spread": [info["spread_a"], info["spread_b"]], "steps": steps,
+                         "engine": {1: "rocblas", 2: "mfma"}.get(info["backend"], "vector")}
         log(f"{path}: {ms:.2f} ms/step, {eff_flops / ms / 1e6:.1f} GFLOP/s effective, "
             f"stages {np.round(stage, 3).tolist()} ms")
     mdg.set_profiling(False)
+    mdg.set_int8_backend(0)
 
     # exact normwise error of every path on sampled entries
     import pyoracle as orc  # noqa: E402  (the checker)
@@ -131,7 +137,7 @@ def main():
     if head_path == "int8":
         work = P * 2.0 * m * n * k
         peak, unit = INT8_PEAK_TOPS, "TFLOP/s"
-        kernel = "diag_i8_kernel"
+        kernel = "rocblas_int8_gemm" if head["engine"] == "rocblas" else "diag_i8_kernel"
     elif head_path == "bf16":
         work = P * 2.0 * m * n * k
         peak, unit = BF16_PEAK_TFLOPS, "TFLOP/s"

@@ -64,6 +64,8 @@ typedef struct mdg_gemm_info {
        profiling is on (mdg_set_profiling), else 0:
        [0] exponent scan, [1] slicing, [2] diagonal GEMMs, [3] reconstruction */
     float stage_ms[4];
+    int backend;           /* diagonal GEMM engine: 1 rocBLAS (INT8 only),
+                              2 this library's MFMA kernels */
 } mdg_gemm_info;
 
 /* One problem of a batch (all pointers on the device). */
@@ -90,6 +92,11 @@ int mdg_release_workspace(void);
 /* Per-stage HIP-event timing of mdg_gemm (on = 1) into mdg_gemm_info.stage_ms;
    when on, mdg_gemm waits for its own last event before returning. */
 int mdg_set_profiling(int on);
+
+/* Engine of the INT8 diagonal GEMMs: 0 = default (rocBLAS), 1 = rocBLAS
+   gemm_ex (INT8 x INT8 -> INT32), 2 = this library's MFMA kernel.  Both are
+   exact; they differ only in speed.  Process-wide. */
+int mdg_set_int8_backend(int backend);
 
 /* ----------------------------------------------------------------------
  * Vector GEMM.  C = A * B with one thread per output entry, each a k-step

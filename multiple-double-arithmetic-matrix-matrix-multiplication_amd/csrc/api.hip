@@ -30,6 +30,10 @@ static std::atomic<int> g_profiling{0};
 
 bool profiling_on() { return g_profiling.load(std::memory_order_relaxed) != 0; }
 
+static std::atomic<int> g_int8_backend{0};
+
+int int8_backend() { return g_int8_backend.load(std::memory_order_relaxed); }
+
 void set_error(const char *fmt, ...)
 {
     va_list ap;
@@ -157,6 +161,13 @@ int mdg_init(void)
 }
 
 int mdg_release_workspace(void) { return release_scratch(); }
+
+int mdg_set_int8_backend(int backend)
+{
+    MDG_REQUIRE(backend >= 0 && backend <= 2, MDG_ERROR_INVALID_VALUE, "unknown INT8 backend %d", backend);
+    g_int8_backend.store(backend);
+    return MDG_SUCCESS;
+}
 
 int mdg_set_profiling(int on)
 {

@@ -56,8 +56,9 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // buffers, so concurrent streams never alias scratch.
 void *scratch(void *stream, size_t bytes, int slot);
 
-// mdg_set_profiling state
+// mdg_set_profiling / mdg_set_int8_backend state
 bool profiling_on();
+int int8_backend();
 
 // HIP events at stage boundaries of one call, on the call's stream
 struct StageTimer {

@@ -32,6 +32,12 @@
 #include <climits>
 #include <cmath>
 
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "common.hpp"
 #include "md_arith.hpp"
 #include "slices.hpp"
@@ -253,76 +259,87 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
 }
 
 // --------------------------------------------------- INT8 diagonal GEMM --
-// Workgroup tile 128 x 128, 4 waves in 2 x 2, each 64 x 64 = 4 x 4 MFMA
-// 16x16x64 tiles; k-block of 128 bytes staged in LDS with rows padded to
-// 144 bytes (conflict-free 16-byte fragment reads).  Buffers are padded so
-// no bounds checks are needed inside the K loop.
-template <bool WIDE>
-__global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__ Aall,
-                                                      const int8_t *__restrict__ Ball, int S, int kp,
-                                                      int m, int n, int tiles_m, int tiles_n,
-                                                      int flush_blocks, void *__restrict__ Dout)
+// INT32 sums of one diagonal are exact while the chunk depth K satisfies
+// K * 2^14 < 2^31, so a diagonal deeper than that is cut into K-chunks, each
+// written to its own INT32 plane; the reconstruction adds the planes of one
+// diagonal in INT64.  Plane p, in ascending diagonal order, is chunk c of
+// diagonal g with nchunks(g) = ceil((g+1) kp / kmax).
+__host__ __device__ inline int i8_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
+
+// deepest diagonal first: z-th plane of that order -> (g, c, first plane of g)
+__device__ inline void i8_plane_of(int z, int S, int kp, int kmax, int &g, int &c)
 {
-    constexpr int BM = 128, BN = 128, BK = 128, LDR = BK + 16;
+    int rem = z;
+    for (g = S - 1; g > 0; --g) {
+        const int nc = i8_chunks(g, kp, kmax);
+        if (rem < nc) break;
+        rem -= nc;
+    }
+    c = rem;
+}
+
+// Hand-written backend: workgroup tile 256 x 128, 8 waves in 4 x 2, each
+// 64 x 64 = 4 x 4 MFMA 16x16x64 tiles; k-block of 128 bytes staged in LDS
+// (rows padded to 144 bytes: conflict-free 16-byte fragment reads); the
+// next k-block is fetched into registers while the current one is
+// multiplied.  Operand buffers are padded (mp % 256, np % 128, kp % 128 == 0),
+// so the K loop carries no bounds checks.
+__global__ __launch_bounds__(512) void diag_i8_kernel(const int8_t *__restrict__ Aall,
+                                                      const int8_t *__restrict__ Ball, int S, int kp, int kmax,
+                                                      int m, int n, int tiles_m, int tiles_n,
+                                                      int *__restrict__ D)
+{
+    constexpr int BM = 256, BN = 128, BK = 128, LDR = BK + 16;
     __shared__ __attribute__((aligned(16))) int8_t As[BM * LDR];
     __shared__ __attribute__((aligned(16))) int8_t Bs[BN * LDR];
 
-    const int g = S - 1 - (int)blockIdx.z;
+    int g, c;
+    i8_plane_of((int)blockIdx.z, S, kp, kmax, g, c);
+    int first = 0;
+    for (int h = 0; h < g; ++h) first += i8_chunks(h, kp, kmax);
+    const int k0 = c * kmax;
+    const int k1 = min((g + 1) * kp, k0 + kmax);
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const long long ld = (long long)S * kp;
-    const int8_t *Ab = Aall + (long long)tm * BM * ld;
-    const int8_t *Bb = Ball + (long long)tn * BN * ld + (long long)(S - 1 - g) * kp;
-    const int Kd = (g + 1) * kp;
+    const int8_t *Ab = Aall + (long long)tm * BM * ld + k0;
+    const int8_t *Bb = Ball + (long long)tn * BN * ld + (long long)(S - 1 - g) * kp + k0;
+    const int Kd = k1 - k0;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lrow = lane & 15, lk = (lane >> 4) * 16;
 
     i4 acc[4][4];
-    long long wide[4][4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc[i][j] = (i4){0, 0, 0, 0};
-            if constexpr (WIDE) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
-            }
-        }
+        for (int j = 0; j < 4; ++j) acc[i][j] = (i4){0, 0, 0, 0};
 
-    // the next k-block is fetched into registers while the current one is
-    // multiplied out of LDS: thread tid moves 16 bytes of rows tid/8 + 32 i
+    // thread tid moves 16 bytes of rows tid/8 + 64 i (A: i < 4, B: i < 2)
     const int frow = tid >> 3, fseg = (tid & 7) * 16;
-    const long long foff = (long long)frow * ld + fseg, fstep = 32 * ld;
-    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define MDG_FETCH_I8(kb)                                      \
-    do {                                                      \
-        ra0 = *(const uint4 *)(Ab + foff + (kb));             \
-        ra1 = *(const uint4 *)(Ab + foff + fstep + (kb));     \
-        ra2 = *(const uint4 *)(Ab + foff + 2 * fstep + (kb)); \
-        ra3 = *(const uint4 *)(Ab + foff + 3 * fstep + (kb)); \
-        rb0 = *(const uint4 *)(Bb + foff + (kb));             \
-        rb1 = *(const uint4 *)(Bb + foff + fstep + (kb));     \
-        rb2 = *(const uint4 *)(Bb + foff + 2 * fstep + (kb)); \
-        rb3 = *(const uint4 *)(Bb + foff + 3 * fstep + (kb)); \
+    const long long foff = (long long)frow * ld + fseg, fstep = 64 * ld;
+    const int soff = frow * LDR + fseg, sstep = 64 * LDR;
+    i4 ra0, ra1, ra2, ra3, rb0, rb1;
+#define MDG_FETCH_I8(kb)                                     \
+    do {                                                     \
+        ra0 = *(const i4 *)(Ab + foff + (kb));               \
+        ra1 = *(const i4 *)(Ab + foff + fstep + (kb));       \
+        ra2 = *(const i4 *)(Ab + foff + 2 * fstep + (kb));   \
+        ra3 = *(const i4 *)(Ab + foff + 3 * fstep + (kb));   \
+        rb0 = *(const i4 *)(Bb + foff + (kb));               \
+        rb1 = *(const i4 *)(Bb + foff + fstep + (kb));       \
     } while (0)
     MDG_FETCH_I8(0);
-    int blk = 0;
-    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
-        *(uint4 *)&As[soff] = ra0;
-        *(uint4 *)&As[soff + sstep] = ra1;
-        *(uint4 *)&As[soff + 2 * sstep] = ra2;
-        *(uint4 *)&As[soff + 3 * sstep] = ra3;
-        *(uint4 *)&Bs[soff] = rb0;
-        *(uint4 *)&Bs[soff + sstep] = rb1;
-        *(uint4 *)&Bs[soff + 2 * sstep] = rb2;
-        *(uint4 *)&Bs[soff + 3 * sstep] = rb3;
+    for (int kb = 0; kb < Kd; kb += BK) {
+        *(i4 *)&As[soff] = ra0;
+        *(i4 *)&As[soff + sstep] = ra1;
+        *(i4 *)&As[soff + 2 * sstep] = ra2;
+        *(i4 *)&As[soff + 3 * sstep] = ra3;
+        *(i4 *)&Bs[soff] = rb0;
+        *(i4 *)&Bs[soff + sstep] = rb1;
         __syncthreads();
         if (kb + BK < Kd) MDG_FETCH_I8(kb + BK);
-        __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 64) {
             i4 a[4], b[4];
@@ -339,21 +356,10 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
                     acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
-        if constexpr (WIDE) {
-            if ((blk + 1) % flush_blocks == 0) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) wide[i][j][v] += acc[i][j][v];
-                        acc[i][j] = (i4){0, 0, 0, 0};
-                    }
-            }
-        }
     }
-    // store D_g: acc element v of tile (i, j) is row 4*(lane/16)+v, col lane%16
-    const long long plane = (long long)g * m * n;
+#undef MDG_FETCH_I8
+    // acc element v of tile (i, j) is row 4*(lane/16)+v, col lane%16
+    int *plane = D + (long long)(first + c) * m * n;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -362,12 +368,7 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long row = (long long)tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
-                if (row < m && col < n) {
-                    if constexpr (WIDE)
-                        ((long long *)Dout)[plane + row * n + col] = wide[i][j][v] + acc[i][j][v];
-                    else
-                        ((int *)Dout)[plane + row * n + col] = acc[i][j][v];
-                }
+                if (row < m && col < n) plane[row * n + col] = acc[i][j][v];
             }
         }
 }
@@ -638,6 +639,106 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(int m, int n, int S, i
     for (int l = 0; l < L; ++l) o[l] = y[l];
 }
 
+// INT8 planes: the chunks of one diagonal are summed exactly in INT64 first
+template <int L>
+__global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S, int kp, int kmax, int w,
+                                                             const int *__restrict__ D, const int *__restrict__ E,
+                                                             const int *__restrict__ F, double *__restrict__ C,
+                                                             long long ldc)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * n) return;
+    const long long r = idx / n, c = idx - r * n;
+    double y[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    const int er = E[r], fc = F[c];
+    if (er != ZERO_SCALE && fc != ZERO_SCALE) {
+        const long long mn = (long long)m * n;
+        int first = 0;
+        for (int g = 0; g < S; ++g) {
+            const int nc = i8_chunks(g, kp, kmax);
+            long long v = 0;
+            for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
+            first += nc;
+            const int sc = er + fc + 4 - w * (g + 2);
+            const long long hi = v & ~0xFFFFFFFFll;
+            const long long lo = v - hi;
+            if (hi != 0) md_inc_d<L>(y, ldexp((double)hi, sc));
+            if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
+        }
+    }
+    double *o = C + (r * ldc + c) * L;
+#pragma unroll
+    for (int l = 0; l < L; ++l) o[l] = y[l];
+}
+
+// ------------------------------------------------------- rocBLAS backend --
+// The diagonal GEMMs are plain INT8 x INT8 -> INT32 GEMMs; rocBLAS runs each
+// K-chunk plane as one gemm_ex call (column-major view: D_p^T (n x m) =
+// Bt_chunk^T-op (n x K) * A_chunk (K x m), lda = ldb = S*kp).
+struct BlasHandle {
+    rocblas_handle h = nullptr;
+    std::mutex mu;
+};
+static std::mutex g_blas_mu;
+static std::map<int, BlasHandle *> g_blas;
+
+static BlasHandle *blas_handle()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_blas_mu);
+    auto it = g_blas.find(dev);
+    if (it != g_blas.end()) return it->second;
+    BlasHandle *b = new BlasHandle();
+    if (rocblas_create_handle(&b->h) != rocblas_status_success) {
+        delete b;
+        return nullptr;
+    }
+    g_blas[dev] = b;
+    return b;
+}
+
+static int int8_backend_choice()
+{
+    // 0 auto (rocBLAS), 1 rocBLAS, 2 hand-written MFMA kernel
+    return int8_backend();
+}
+
+static int i8_planes_rocblas(const int8_t *Aall, const int8_t *Ball, int S, int kp, int kmax, int m, int n, int *D,
+                             hipStream_t s)
+{
+    BlasHandle *b = blas_handle();
+    MDG_REQUIRE(b, MDG_ERROR_HIP, "rocblas_create_handle failed");
+    std::lock_guard<std::mutex> lock(b->mu);
+    if (rocblas_set_stream(b->h, s) != rocblas_status_success) {
+        set_error("rocblas_set_stream failed");
+        return MDG_ERROR_HIP;
+    }
+    const int32_t alpha = 1, beta = 0;
+    const long long ld = (long long)S * kp;
+    const long long mn = (long long)m * n;
+    int p = 0;
+    for (int g = 0; g < S; ++g) {
+        const int K = (g + 1) * kp;
+        for (int k0 = 0; k0 < K; k0 += kmax, ++p) {
+            const int kk = std::min(kmax, K - k0);
+            int *Dp = D + (long long)p * mn;
+            rocblas_status st = rocblas_gemm_ex(
+                b->h, rocblas_operation_transpose, rocblas_operation_none, n, m, kk, &alpha,
+                Ball + (long long)(S - 1 - g) * kp + k0, rocblas_datatype_i8_r, ld, Aall + k0, rocblas_datatype_i8_r,
+                ld, &beta, Dp, rocblas_datatype_i32_r, n, Dp, rocblas_datatype_i32_r, n, rocblas_datatype_i32_r,
+                rocblas_gemm_algo_standard, 0, 0);
+            if (st != rocblas_status_success) {
+                set_error("rocblas_gemm_ex (int8) failed with status %d", (int)st);
+                return MDG_ERROR_HIP;
+            }
+        }
+    }
+    return MDG_SUCCESS;
+}
+
 // ---------------------------------------------------------------- driver --
 static inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
 
@@ -651,6 +752,7 @@ struct Plan {
     bool wide;        // INT8: widened accumulation
     int flush_blocks; // FP64 / INT8
     int spread_a, spread_b;
+    int backend;      // INT8: 1 rocBLAS, 2 hand-written MFMA kernel
 };
 
 // Choose slice width and count for a precision target and the measured
@@ -749,34 +851,38 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     MDG_REQUIRE(zdim <= 65535u, MDG_ERROR_RANGE, "too many slices (%d)", S);
 
     if (p.method == MDG_METHOD_INT8_OZAKI) {
-        // exact INT32 while (g+1)*kp * 2^14 < 2^31
-        p.wide = ((long long)S * kp > 131072ll - 128);
-        p.flush_blocks = (int)((131072ll - 128) / 128);
-        int8_t *Aall = (int8_t *)scratch(s, (size_t)mp * S * kp, 2);
+        // INT32 planes exact while chunk depth K * 2^14 < 2^31
+        const int kmax = (kp <= 131071) ? (131071 / kp) * kp : 130944;
+        int nplanes = 0;
+        for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
+        p.wide = nplanes > S;
+        p.flush_blocks = kmax;
+        const int mp8 = (int)round_up(m, 256);
+        int8_t *Aall = (int8_t *)scratch(s, (size_t)mp8 * S * kp, 2);
         int8_t *Ball = (int8_t *)scratch(s, (size_t)np * S * kp, 3);
-        void *D = scratch(s, (size_t)S * mn * (p.wide ? 8 : 4), 4);
+        int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
         MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-        hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp * kp, 256)), dim3(256), 0,
-                           s, m, k, mp, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp8 * kp, 256)), dim3(256), 0,
+                           s, m, k, mp8, kp, A, lda, E, S, w, Aall);
         hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
         tm.mark(2);
-        const int tiles_n = np / 128;
-        dim3 grid(tiles_m * tiles_n, 1, zdim);
-        if (p.wide)
-            hipLaunchKernelGGL((diag_i8_kernel<true>), grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m, tiles_n,
-                               p.flush_blocks, D);
-        else
-            hipLaunchKernelGGL((diag_i8_kernel<false>), grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m,
-                               tiles_n, p.flush_blocks, D);
-        MDG_LAUNCH_CHECK("diag_i8_kernel");
+        const int be = int8_backend_choice();
+        p.backend = (be == 2) ? 2 : 1;
+        if (p.backend == 1) {
+            rc = i8_planes_rocblas(Aall, Ball, S, kp, kmax, m, n, D, s);
+            if (rc) return rc;
+        } else {
+            MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
+            const int tiles_m8 = mp8 / 256, tiles_n = np / 128;
+            dim3 grid(tiles_m8 * tiles_n, 1, (unsigned)nplanes);
+            hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(512), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
+                               tiles_n, D);
+            MDG_LAUNCH_CHECK("diag_i8_kernel");
+        }
         tm.mark(3);
-        if (p.wide)
-            hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m,
-                               n, S, w, (const long long *)D, E, F, C, ldc);
-        else
-            hipLaunchKernelGGL((reconstruct_kernel<L, int>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S,
-                               w, (const int *)D, E, F, C, ldc);
+        hipLaunchKernelGGL((reconstruct_i8_kernel<L>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, kp,
+                           kmax, w, (const int *)D, E, F, C, ldc);
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
         p.wide = true;
         p.flush_blocks = 16;
@@ -831,6 +937,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         info->spread_a = p.spread_a;
         info->spread_b = p.spread_b;
         info->flushed = p.wide ? 1 : 0;
+        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? p.backend : 2;
         for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;

@@ -32,7 +32,7 @@ class GemmInfo(ctypes.Structure):
                 ("diagonals", ctypes.c_int), ("pair_gemms", ctypes.c_longlong),
                 ("total_bits", ctypes.c_int), ("spread_a", ctypes.c_int),
                 ("spread_b", ctypes.c_int), ("flushed", ctypes.c_int),
-                ("stage_ms", ctypes.c_float * 4)]
+                ("stage_ms", ctypes.c_float * 4), ("backend", ctypes.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -56,6 +56,7 @@ _SIGS = {
     "mdg_init": ([], _i),
     "mdg_release_workspace": ([], _i),
     "mdg_set_profiling": ([_i], _i),
+    "mdg_set_int8_backend": ([_i], _i),
     "mdg_vector_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _vp], _i),
     "mdg_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _i, ctypes.POINTER(GemmInfo), _vp], _i),
     "mdg_gemm_batched": ([_i, ctypes.POINTER(Problem), _i, _i, _i, _i, _vp], _i),
@@ -112,6 +113,11 @@ def _stream(stream):
 # --------------------------------------------------------------- wrappers --
 def set_profiling(on):
     _check(lib().mdg_set_profiling(1 if on else 0))
+
+
+def set_int8_backend(backend):
+    """0 default (rocBLAS), 1 rocBLAS, 2 hand-written MFMA kernel."""
+    _check(lib().mdg_set_int8_backend(int(backend)))
 
 
 def empty(n, device="cuda"):

@@ -8,7 +8,19 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "bf16"]
+METHODS = ["f64", "int8", "int8_mfma", "bf16"]
+
+
+def gemm(mdg, L, m, n, k, A, B, method, **kw):
+    """int8 runs on rocBLAS (the default engine), int8_mfma on this library's
+    INT8 MFMA kernel; both must give the same exact diagonal sums."""
+    if method == "int8_mfma":
+        mdg.set_int8_backend(2)
+        try:
+            return mdg.gemm(L, m, n, k, A, B, method="int8", **kw)
+        finally:
+            mdg.set_int8_backend(0)
+    return mdg.gemm(L, m, n, k, A, B, method=method, **kw)
 
 
 def _bound(L, k):
@@ -23,7 +35,7 @@ def test_split_gemm_error_bound(mdg, orc, torch_cuda, method, L):
         print(f"split {method} L={L} {m}x{n}x{k} e=[{emin},{emax}]", flush=True)
         A = orc.generate(orc.GEN_QD, L, m, k, 7 * L + m, emin, emax)
         B = orc.generate(orc.GEN_QD, L, k, n, 9 * L + n, emin, emax)
-        C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method=method)
+        C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
         C = to_host(C)
         err, _ = orc.normwise_error(L, m, n, k, A, B, C)
         assert err <= _bound(L, k), f"{method} L={L}: err={err:.3e} bound={_bound(L, k):.3e} {info}"
@@ -39,7 +51,7 @@ def test_split_gemm_dd_config1_inputs(mdg, orc, torch_cuda, method):
     L, n = 2, 128
     A = orc.generate(orc.GEN_DD, L, n, n, 1)
     B = orc.generate(orc.GEN_DD, L, n, n, 2)
-    C, info = mdg.gemm(L, n, n, n, to_dev(A), to_dev(B), method=method)
+    C, info = gemm(mdg, L, n, n, n, to_dev(A), to_dev(B), method)
     err, _ = orc.normwise_error(L, n, n, n, A, B, to_host(C))
     assert err <= _bound(L, n), (err, info)
 
@@ -51,7 +63,7 @@ def test_split_gemm_zero_rows_and_columns(mdg, orc, torch_cuda, method):
     B = orc.generate(orc.GEN_QD, L, k, n, 4, -8, 8).reshape(k, n, L)
     A[3] = 0.0
     B[:, 5] = 0.0
-    C, _ = mdg.gemm(L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method=method)
+    C, _ = gemm(mdg, L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method)
     C = to_host(C).reshape(m, n, L)
     assert np.all(C[3] == 0.0) and np.all(C[:, 5] == 0.0)
     err, _ = orc.normwise_error(L, m, n, k, A.ravel(), B.ravel(), C.ravel())
@@ -62,7 +74,7 @@ def test_split_gemm_zero_rows_and_columns(mdg, orc, torch_cuda, method):
 def test_split_gemm_all_zero(mdg, torch_cuda, method):
     L, m, n, k = 2, 9, 7, 5
     z = to_dev(np.zeros(max(m, n) * k * L))
-    C, _ = mdg.gemm(L, m, n, k, z, z, method=method, C=to_dev(np.ones(m * n * L)))
+    C, _ = gemm(mdg, L, m, n, k, z, z, method, C=to_dev(np.ones(m * n * L)))
     assert np.all(to_host(C) == 0.0)
 
 
@@ -75,7 +87,7 @@ def test_split_gemm_small_integers_exact(mdg, torch_cuda, method):
     B = np.zeros((k, n, L))
     A[:, :, 0] = rng.integers(-100, 101, (m, k))
     B[:, :, 0] = rng.integers(-100, 101, (k, n))
-    C, _ = mdg.gemm(L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method=method)
+    C, _ = gemm(mdg, L, m, n, k, to_dev(A.ravel()), to_dev(B.ravel()), method)
     C = to_host(C).reshape(m, n, L)
     assert np.array_equal(C[:, :, 0], A[:, :, 0] @ B[:, :, 0])
     assert np.all(C[:, :, 1:] == 0.0)
@@ -90,12 +102,27 @@ def test_split_gemm_rejects_non_finite(mdg, torch_cuda):
     assert ei.value.status == 5
 
 
-def test_int8_widened_accumulation(mdg, orc, torch_cuda):
-    # S * kp > 2^17: the INT8 kernel must widen INT32 sums to INT64 in-loop
+@pytest.mark.parametrize("method", ["int8", "int8_mfma"])
+def test_int8_chunked_diagonals(mdg, orc, torch_cuda, method):
+    # S * kp > 2^17: the deepest diagonals are cut into K-chunks whose INT32
+    # planes are added in INT64 by the reconstruction
     L, m, n, k = 4, 64, 64, 4608
     A = orc.generate(orc.GEN_QD, L, m, k, 5, -8, 8)
     B = orc.generate(orc.GEN_QD, L, k, n, 6, -8, 8)
-    C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method="int8")
+    C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
     assert info["flushed"] == 1, info
     err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=512)
     assert err <= _bound(L, k), (err, info)
+
+
+@pytest.mark.parametrize("L", [2, 4])
+def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
+    # rocBLAS and the MFMA kernel compute the same exact diagonal sums, so the
+    # reconstructed results must be identical bit for bit
+    m, n, k = 300, 260, 333
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 17, -20, 20))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 18, -20, 20))
+    C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8")
+    C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
+    assert i1["backend"] == 1 and i2["backend"] == 2
+    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
