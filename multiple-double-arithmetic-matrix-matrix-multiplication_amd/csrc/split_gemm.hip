@@ -478,21 +478,25 @@ __global__ __launch_bounds__(256) void diag_bf16_kernel(const __bf16 *__restrict
 }
 
 // --------------------------------------------------- FP64 diagonal GEMM --
-// Workgroup tile 128 x 64, 4 waves in 2 x 2, each 64 x 32 = 4 x 2 MFMA
-// 16x16x4 tiles; k-block of 16 staged in LDS as doubles (converted from the
-// INT32 slices on the way in, halving the global traffic).
+// Workgroup tile 128 x 128, 8 waves in 2 x 4, each 64 x 32 = 4 x 2 MFMA
+// 16x16x4 tiles; k-block of 32 staged in LDS as doubles (converted from the
+// INT32 slices on the way in, halving the global traffic), fetched one
+// k-block ahead into registers.  (76% of the FP64 matrix peak at the quad-
+// double n = 4096 shape, against 52% for a 4-wave 128 x 64 tile.)
 //   OUT 0: store FP64 sums          (limb convolution of the reference path)
 //   OUT 1: add FP64 sums into C     (the reference's accumulate GEMM)
-//   OUT 2: store INT64, widening every flush_blocks k-blocks (split path)
+//   OUT 2: store INT64, moving the FP64 sums into INT64 every `flushk` k
+//          (split path; the FP64 partial sums stay exact below 2^53)
 // TA is the stored slice type (int32_t or double).  All loads are
 // bounds-checked against (rows, Kd) so unpadded operands work.
 template <typename TA, int OUT>
-__global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aall, const TA *__restrict__ Ball,
+__global__ __launch_bounds__(512) void diag_f64_kernel(const TA *__restrict__ Aall, const TA *__restrict__ Ball,
                                                        int S, int kp, long long lda, long long ldb, int m, int n,
-                                                       int tiles_m, int tiles_n, int flush_blocks,
+                                                       int tiles_m, int tiles_n, int flushk,
                                                        void *__restrict__ Dout, long long ldd)
 {
-    constexpr int BM = 128, BN = 64, BK = 16, LDR = BK + 1;
+    constexpr int NT = 512, BM = 128, BN = 128, BK = 32, LDR = BK + 1;
+    constexpr int PA = BM * BK / NT, PB = BN * BK / NT;   // 8 each
     __shared__ double As[BM * LDR];
     __shared__ double Bs[BN * LDR];
 
@@ -505,7 +509,7 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
     const int Kd = (g + 1) * kp;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave >> 2, wn = wave & 3;
     const int lrow = lane & 15, lk = lane >> 4;
 
     d4 acc[4][2];
@@ -519,40 +523,44 @@ __global__ __launch_bounds__(256) void diag_f64_kernel(const TA *__restrict__ Aa
             for (int v = 0; v < 4; ++v) wide[i][j][v] = 0;
         }
 
-    // A: 128 rows x 16 k = 2048 values, 8 per thread; B: 64 x 16 = 1024, 4
-    // per thread; fetched one k-block ahead into registers
-    TA ra[8], rb[4];   // raw slices; converted to double on the way into LDS
-#define MDG_FETCH3(kb) \
-    do { \
-_Pragma("unroll") \
-        for (int it = 0; it < 8; ++it) { \
-            const int idx = tid + it * 256; \
-            const int row = idx >> 4, kk = idx & 15; \
-            ra[it] = (row0 + row < m && kb + kk < Kd) ? Ab[(long long)row * lda + kb + kk] : (TA)0; \
-        } \
-_Pragma("unroll") \
-        for (int it = 0; it < 4; ++it) { \
-            const int idx = tid + it * 256; \
-            const int row = idx >> 4, kk = idx & 15; \
-            rb[it] = (col0 + row < n && kb + kk < Kd) ? Bb[(long long)row * ldb + kb + kk] : (TA)0; \
-        } \
-    } while (0)
-    MDG_FETCH3(0);
-    int blk = 0;
-    for (int kb = 0; kb < Kd; kb += BK, ++blk) {
+    // element idx = tid + it*512 of a tile is (row idx/32, k idx%32)
+    TA ra[PA], rb[PB];   // raw slices; converted to double on the way into LDS
 #pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int idx = tid + it * 256;
-            As[(idx >> 4) * LDR + (idx & 15)] = (double)ra[it];
+    for (int it = 0; it < PA; ++it) {
+        const int idx = tid + it * NT, row = idx >> 5, kk = idx & 31;
+        ra[it] = (row0 + row < m && kk < Kd) ? Ab[(long long)row * lda + kk] : (TA)0;
+    }
+#pragma unroll
+    for (int it = 0; it < PB; ++it) {
+        const int idx = tid + it * NT, row = idx >> 5, kk = idx & 31;
+        rb[it] = (col0 + row < n && kk < Kd) ? Bb[(long long)row * ldb + kk] : (TA)0;
+    }
+    int kacc = 0;
+    for (int kb = 0; kb < Kd; kb += BK) {
+#pragma unroll
+        for (int it = 0; it < PA; ++it) {
+            const int idx = tid + it * NT;
+            As[(idx >> 5) * LDR + (idx & 31)] = (double)ra[it];
         }
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int idx = tid + it * 256;
-            Bs[(idx >> 4) * LDR + (idx & 15)] = (double)rb[it];
+        for (int it = 0; it < PB; ++it) {
+            const int idx = tid + it * NT;
+            Bs[(idx >> 5) * LDR + (idx & 31)] = (double)rb[it];
         }
         __syncthreads();
-        if (kb + BK < Kd) MDG_FETCH3(kb + BK);
-        __syncthreads();
+        if (kb + BK < Kd) {
+            const int kn = kb + BK;
+#pragma unroll
+            for (int it = 0; it < PA; ++it) {
+                const int idx = tid + it * NT, row = idx >> 5, kk = idx & 31;
+                ra[it] = (row0 + row < m && kn + kk < Kd) ? Ab[(long long)row * lda + kn + kk] : (TA)0;
+            }
+#pragma unroll
+            for (int it = 0; it < PB; ++it) {
+                const int idx = tid + it * NT, row = idx >> 5, kk = idx & 31;
+                rb[it] = (col0 + row < n && kn + kk < Kd) ? Bb[(long long)row * ldb + kn + kk] : (TA)0;
+            }
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
             double a[4], b[2];
@@ -568,7 +576,9 @@ _Pragma("unroll") \
         }
         __syncthreads();
         if constexpr (OUT == 2) {
-            if ((blk + 1) % flush_blocks == 0) {
+            kacc += BK;
+            if (kacc >= flushk) {
+                kacc = 0;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -604,6 +614,23 @@ _Pragma("unroll") \
 }
 
 // ------------------------------------------------------------ reconstruct --
+// y += v * 2^sc for an exact integer v: one inc_d when v fits a double's
+// 53 bits, else two (a 32-bit-aligned high part and the low remainder)
+template <int L>
+__device__ __forceinline__ void inc_int64(double *y, long long v, int sc)
+{
+    if (v == 0) return;
+    if (v > -(1ll << 53) && v < (1ll << 53)) {
+        md_inc_d<L>(y, ldexp((double)v, sc));
+    } else {
+        const long long hi = v & ~0xFFFFFFFFll;
+        const long long lo = v - hi;   // in [0, 2^32)
+        md_inc_d<L>(y, ldexp((double)hi, sc));
+        if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
+    }
+}
+
+
 // C[r][c] = fold over g of inc_d(C, D_g[r][c] * 2^(E_r + F_c + 4 - w(g+2))),
 // starting from zero, g = 0 (most significant) first.  An INT64 diagonal
 // sum is split into two exact doubles (high 32-bit-aligned part, low part).
@@ -623,15 +650,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(int m, int n, int S, i
         const long long mn = (long long)m * n;
         for (int g = 0; g < S; ++g) {
             const int sc = er + fc + 4 - w * (g + 2);
-            const PT v = D[(long long)g * mn + idx];
-            if constexpr (sizeof(PT) == 8) {
-                const long long hi = v & ~0xFFFFFFFFll;
-                const long long lo = v - hi;   // in [0, 2^32)
-                if (hi != 0) md_inc_d<L>(y, ldexp((double)hi, sc));
-                if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
-            } else {
-                if (v != 0) md_inc_d<L>(y, ldexp((double)v, sc));
-            }
+            inc_int64<L>(y, (long long)D[(long long)g * mn + idx], sc);
         }
     }
     double *o = C + (r * ldc + c) * L;
@@ -662,10 +681,7 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
             for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
             first += nc;
             const int sc = er + fc + 4 - w * (g + 2);
-            const long long hi = v & ~0xFFFFFFFFll;
-            const long long lo = v - hi;
-            if (hi != 0) md_inc_d<L>(y, ldexp((double)hi, sc));
-            if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
+            inc_int64<L>(y, v, sc);
         }
     }
     double *o = C + (r * ldc + c) * L;
@@ -775,7 +791,7 @@ static int plan_slices(Plan &p, int target_bits)
         // FP64: widest w with (2w-2) + log2(S * kp) <= 62 (INT64 diagonal
         // sums) and one 16-deep k-block exact in FP64 (16 * 2^(2w-2) <= 2^53)
         p.w = 0;
-        for (int w = 25; w >= 8; --w) {
+        for (int w = 25; w >= 8; --w) {   // 2^(55-2w) >= 32: one k-block exact
             int S = 2;
             while (S * w < need_W(S)) ++S;
             const double bits = (2.0 * w - 2.0) + std::log2((double)S * p.kp);
@@ -787,8 +803,8 @@ static int plan_slices(Plan &p, int target_bits)
         }
         if (p.w == 0) return MDG_ERROR_RANGE;
         // FP64 partial sums exact while count * 2^(2w-2) <= 2^53
-        const long long chunk = 1ll << (55 - 2 * p.w);
-        p.flush_blocks = (int)std::max(1ll, chunk / 16);
+        // (for FP64, flush_blocks holds the widening period in k)
+        p.flush_blocks = (int)(1ll << (55 - 2 * p.w));
     }
     p.W = p.S * p.w;
     return MDG_SUCCESS;
@@ -913,9 +929,9 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         hipLaunchKernelGGL((slice_b_kernel<L, int32_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
         tm.mark(2);
-        const int tiles_n = np / 64;
+        const int tiles_n = np / 128;
         dim3 grid(tiles_m * tiles_n, 1, zdim);
-        hipLaunchKernelGGL((diag_f64_kernel<int32_t, 2>), grid, dim3(256), 0, s, Aall, Ball, S, kp,
+        hipLaunchKernelGGL((diag_f64_kernel<int32_t, 2>), grid, dim3(512), 0, s, Aall, Ball, S, kp,
                            (long long)S * kp, (long long)S * kp, m, n, tiles_m, tiles_n, p.flush_blocks, (void *)D,
                            (long long)n);
         MDG_LAUNCH_CHECK("diag_f64_kernel");
@@ -969,10 +985,10 @@ int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const
 
 int dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, hipStream_t s)
 {
-    const int tiles_m = (M + 127) / 128, tiles_n = (N + 63) / 64;
+    const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
     dim3 grid(tiles_m * tiles_n, 1, 1);
     // one "diagonal" (S = 1) of depth K; B is column-major = K-contiguous rows
-    hipLaunchKernelGGL((diag_f64_kernel<double, 1>), grid, dim3(256), 0, s, A, B, 1, K, (long long)K, (long long)K, M,
+    hipLaunchKernelGGL((diag_f64_kernel<double, 1>), grid, dim3(512), 0, s, A, B, 1, K, (long long)K, (long long)K, M,
                        N, tiles_m, tiles_n, 1, (void *)C, (long long)N);
     MDG_LAUNCH_CHECK("diag_f64_kernel (dgemm_acc)");
     return MDG_SUCCESS;
@@ -1030,9 +1046,9 @@ static int conv_planes(int m, int n, int k, int pp, const double *Aq, const doub
     // Bq[kk][c][t]: "row" c has stride pp, kk has stride n*pp
     hipLaunchKernelGGL(parts_to_planes_kernel, dim3(blocks_for<int>(nb, 256)), dim3(256), 0, s, n, k, pp, Bq,
                        (long long)pp, (long long)n * pp, 1, Bp);
-    const int tiles_m = (m + 127) / 128, tiles_n = (n + 63) / 64;
+    const int tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
     dim3 grid(tiles_m * tiles_n, 1, (unsigned)pp);
-    hipLaunchKernelGGL((diag_f64_kernel<double, 0>), grid, dim3(256), 0, s, Ap, Bp, pp, k, (long long)pp * k,
+    hipLaunchKernelGGL((diag_f64_kernel<double, 0>), grid, dim3(512), 0, s, Ap, Bp, pp, k, (long long)pp * k,
                        (long long)pp * k, m, n, tiles_m, tiles_n, 1, (void *)D, (long long)n);
     MDG_LAUNCH_CHECK("diag_f64_kernel (conv)");
     return MDG_SUCCESS;
