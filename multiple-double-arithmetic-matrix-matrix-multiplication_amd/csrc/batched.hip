@@ -1,13 +1,21 @@
 // batched.hip -- many independent multi-double GEMMs of uneven shape.
 //
-// persistent = 0: a loop of per-problem launches on the caller's stream
-//                 (the baseline a caller of the reference would write around
+// persistent = 0: a loop of per-problem calls on the caller's stream (the
+//                 baseline a caller of the reference would write around
 //                 matmulTCnt, C++/TCcuda.cu:248-302).
-// persistent = 1: the same work recorded once into a hipGraph keyed by the
-//                 batch's shape signature and replayed on later calls, so the
-//                 per-launch CPU cost is paid once.
-// Slice counts come from the caller's spread allowance, never from a
-// device->host read, so nothing in the batch synchronises the stream.
+// persistent = 1: the whole batch as one pipeline of stage launches, each
+//                 launch covering every problem through a device table of
+//                 (problem, block) entries: exponent scans, slicing, ONE
+//                 launch of INT8 diagonal-GEMM tiles of all problems (or one
+//                 vector-GEMM launch per limb count), reconstruction.  The
+//                 pipeline is recorded once into a hipGraph keyed by the
+//                 batch's shape signature and replayed on later calls.
+// Slice counts come from the caller's spread allowance (no device->host
+// read), so nothing in the batch synchronises the stream.  Methods other
+// than INT8 and the vector GEMM run the per-problem loop inside the graph.
+#include <algorithm>
+#include <climits>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -15,6 +23,8 @@
 #include <vector>
 
 #include "common.hpp"
+#include "md_arith.hpp"
+#include "slices.hpp"
 
 namespace mdg {
 
@@ -22,6 +32,529 @@ int vector_gemm(int L, int m, int n, int k, const double *A, long long lda, cons
                 double *C, long long ldc, int twoprod, hipStream_t s);
 int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
                long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s);
+int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b);
+
+typedef int i4 __attribute__((ext_vector_type(4)));
+constexpr int BZERO_SCALE = -4000;
+
+// one problem of the fused pipeline, as the kernels see it
+struct BProb {
+    int L, m, n, k, S, kp, mp, np, kmax, nplanes, w;
+    long long lda, ldb, ldc;
+    const double *A, *B;
+    double *C;
+    int8_t *Aall, *Ball;
+    int *D, *E, *F;
+    double *colsum;
+};
+
+// one block's work: problem and up to three block coordinates
+struct BEnt {
+    int prob, x, y, z;
+};
+
+__device__ __forceinline__ int bscale_exponent(double s)
+{
+    if (s == 0.0) return INT_MIN;
+    int ex;
+    frexp(s * (1.0 + 0x1p-40), &ex);
+    return ex;
+}
+
+// ---------------------------------------------------------------- scans --
+__global__ void b_init_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const int c = e.x * 256 + threadIdx.x;
+    if (c < p.n) {
+        p.F[c] = INT_MIN;
+        p.colsum[c] = 0.0;
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void b_scan_rows_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const int lane = threadIdx.x & 63;
+    const long long r = (long long)e.x * 4 + (threadIdx.x >> 6);
+    if (r >= p.m) return;
+    int emax = INT_MIN;
+    for (int kk = lane; kk < p.k; kk += 64) {
+        const double *x = p.A + (r * p.lda + kk) * L;
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) s += fabs(x[l]);
+        emax = max(emax, bscale_exponent(s));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) emax = max(emax, __shfl_xor(emax, o));
+    if (lane == 0) p.E[r] = (emax == INT_MIN) ? BZERO_SCALE : emax;
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void b_scan_cols_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    __shared__ int se[4][64];
+    const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const long long c = (long long)e.x * 64 + cl;
+    const int k0 = e.y * 256, k1 = min(p.k, k0 + 256);
+    int emax = INT_MIN;
+    if (c < p.n) {
+        for (int kk = k0 + grp; kk < k1; kk += 4) {
+            const double *x = p.B + ((long long)kk * p.ldb + c) * L;
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < L; ++l) s += fabs(x[l]);
+            emax = max(emax, bscale_exponent(s));
+        }
+    }
+    se[grp][cl] = emax;
+    __syncthreads();
+    if (grp == 0 && c < p.n) {
+        const int m4 = max(max(se[0][cl], se[1][cl]), max(se[2][cl], se[3][cl]));
+        if (m4 != INT_MIN) atomicMax(&p.F[c], m4);
+    }
+}
+
+__global__ void b_finish_cols_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const int c = e.x * 256 + threadIdx.x;
+    if (c < p.n && p.F[c] == INT_MIN) p.F[c] = BZERO_SCALE;
+}
+
+// --------------------------------------------------------------- slices --
+template <int L>
+__global__ __launch_bounds__(256) void b_slice_a_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    constexpr int NW = FixedWords<L>::NW;
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const long long idx = (long long)e.x * 256 + threadIdx.x;
+    if (idx >= (long long)p.mp * p.kp) return;
+    const long long r = idx / p.kp;
+    const int kk = (int)(idx - r * p.kp);
+    int8_t *o = p.Aall + r * ((long long)p.S * p.kp) + kk;
+    if (r >= p.m || kk >= p.k || p.E[r] == BZERO_SCALE) {
+        for (int t = 0; t < p.S; ++t) o[(long long)t * p.kp] = 0;
+        return;
+    }
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = p.A[(r * p.lda + kk) * L + l];
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, p.S * p.w - 2 - p.E[r], N);
+    fixed_digits<NW>(N, p.S, p.w, [&](int t, int d) { o[(long long)t * p.kp] = (int8_t)d; });
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void b_slice_b_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    constexpr int NW = FixedWords<L>::NW;
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const long long idx = (long long)e.x * 256 + threadIdx.x;
+    if (idx >= (long long)p.np * p.kp) return;
+    const long long c = idx / p.kp;
+    const int kk = (int)(idx - c * p.kp);
+    int8_t *o = p.Ball + c * ((long long)p.S * p.kp) + kk;
+    if (c >= p.n || kk >= p.k || p.F[c] == BZERO_SCALE) {
+        for (int u = 0; u < p.S; ++u) o[(long long)u * p.kp] = 0;
+        return;
+    }
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = p.B[((long long)kk * p.ldb + c) * L + l];
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, p.S * p.w - 2 - p.F[c], N);
+    fixed_digits<NW>(N, p.S, p.w, [&](int u, int d) { o[(long long)(p.S - 1 - u) * p.kp] = (int8_t)d; });
+}
+
+// --------------------------------------------------- INT8 diagonal tiles --
+// Entry (prob, tile_m, tile_n, plane): a 128 x 128 tile (4 waves, each
+// 64 x 64 of MFMA 16x16x64) of one K-chunk plane of one diagonal.  Plane
+// numbering as in the single-problem path: ascending diagonal g, chunk c.
+__device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
+
+__global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    constexpr int BM = 128, BN = 128, BK = 128, LDR = BK + 16;
+    __shared__ __attribute__((aligned(16))) int8_t As[BM * LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[BN * LDR];
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const int S = p.S, kp = p.kp, kmax = p.kmax;
+    // plane e.z -> (g, c)
+    int g = 0, first = 0;
+    while (first + b_chunks(g, kp, kmax) <= e.z) {
+        first += b_chunks(g, kp, kmax);
+        ++g;
+    }
+    const int c = e.z - first;
+    const int k0 = c * kmax, k1 = min((g + 1) * kp, k0 + kmax);
+    const long long ld = (long long)S * kp;
+    const int8_t *Ab = p.Aall + (long long)e.x * BM * ld + k0;
+    const int8_t *Bb = p.Ball + (long long)e.y * BN * ld + (long long)(S - 1 - g) * kp + k0;
+    const int Kd = k1 - k0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = lane & 15, lk = (lane >> 4) * 16;
+    i4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (i4){0, 0, 0, 0};
+    const int frow = tid >> 3, fseg = (tid & 7) * 16;
+    const long long foff = (long long)frow * ld + fseg, fstep = 32 * ld;
+    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
+    i4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define MDG_BFETCH(kb)                                       \
+    do {                                                     \
+        ra0 = *(const i4 *)(Ab + foff + (kb));               \
+        ra1 = *(const i4 *)(Ab + foff + fstep + (kb));       \
+        ra2 = *(const i4 *)(Ab + foff + 2 * fstep + (kb));   \
+        ra3 = *(const i4 *)(Ab + foff + 3 * fstep + (kb));   \
+        rb0 = *(const i4 *)(Bb + foff + (kb));               \
+        rb1 = *(const i4 *)(Bb + foff + fstep + (kb));       \
+        rb2 = *(const i4 *)(Bb + foff + 2 * fstep + (kb));   \
+        rb3 = *(const i4 *)(Bb + foff + 3 * fstep + (kb));   \
+    } while (0)
+    MDG_BFETCH(0);
+    for (int kb = 0; kb < Kd; kb += BK) {
+        *(i4 *)&As[soff] = ra0;
+        *(i4 *)&As[soff + sstep] = ra1;
+        *(i4 *)&As[soff + 2 * sstep] = ra2;
+        *(i4 *)&As[soff + 3 * sstep] = ra3;
+        *(i4 *)&Bs[soff] = rb0;
+        *(i4 *)&Bs[soff + sstep] = rb1;
+        *(i4 *)&Bs[soff + 2 * sstep] = rb2;
+        *(i4 *)&Bs[soff + 3 * sstep] = rb3;
+        __syncthreads();
+        if (kb + BK < Kd) MDG_BFETCH(kb + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 64) {
+            i4 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = *(const i4 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = *(const i4 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#undef MDG_BFETCH
+    int *plane = p.D + (long long)e.z * p.m * p.n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long col = (long long)e.y * BN + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long row = (long long)e.x * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                if (row < p.m && col < p.n) plane[row * p.n + col] = acc[i][j][v];
+            }
+        }
+}
+
+// ------------------------------------------------------- reconstruction --
+template <int L>
+__global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const long long idx = (long long)e.x * 256 + threadIdx.x;
+    if (idx >= (long long)p.m * p.n) return;
+    const long long r = idx / p.n, c = idx - r * p.n;
+    double y[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    const int er = p.E[r], fc = p.F[c];
+    if (er != BZERO_SCALE && fc != BZERO_SCALE) {
+        const long long mn = (long long)p.m * p.n;
+        int first = 0;
+        for (int g = 0; g < p.S; ++g) {
+            const int nc = b_chunks(g, p.kp, p.kmax);
+            long long v = 0;
+            for (int q = 0; q < nc; ++q) v += p.D[(long long)(first + q) * mn + idx];
+            first += nc;
+            if (v == 0) continue;
+            const int sc = er + fc + 4 - p.w * (g + 2);
+            if (v > -(1ll << 53) && v < (1ll << 53)) {
+                md_inc_d<L>(y, ldexp((double)v, sc));
+            } else {
+                const long long hi = v & ~0xFFFFFFFFll, lo = v - hi;
+                md_inc_d<L>(y, ldexp((double)hi, sc));
+                if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
+            }
+        }
+    }
+    double *o = p.C + (r * p.ldc + c) * L;
+#pragma unroll
+    for (int l = 0; l < L; ++l) o[l] = y[l];
+}
+
+// ----------------------------------------------------- fused vector GEMM --
+// Entry (prob, tile_m, tile_n): 16 x 16 outputs of one problem, one thread
+// per output, the same k-chain as vector_gemm.hip (bit-identical results).
+template <int L>
+__global__ __launch_bounds__(256) void b_vector_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    constexpr int TM = 16, TN = 16, TK = 16;
+    __shared__ double As[L][TM][TK + 1];
+    __shared__ double Bs[L][TK][TN + 1];
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    const int tid = threadIdx.x, tx = tid % TN, ty = tid / TN;
+    const long long r0 = (long long)e.x * TM, c0 = (long long)e.y * TN;
+    double acc[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) acc[l] = 0.0;
+    for (int k0 = 0; k0 < p.k; k0 += TK) {
+        for (int idx = tid; idx < TM * TK * L; idx += 256) {
+            const int i = idx / (TK * L), rem = idx - i * (TK * L), kk = rem / L, l = rem - kk * L;
+            const long long gr = r0 + i;
+            const int gk = k0 + kk;
+            As[l][i][kk] = (gr < p.m && gk < p.k) ? p.A[(gr * p.lda + gk) * L + l] : 0.0;
+        }
+        for (int idx = tid; idx < TK * TN * L; idx += 256) {
+            const int kk = idx / (TN * L), rem = idx - kk * (TN * L), j = rem / L, l = rem - j * L;
+            const int gk = k0 + kk;
+            const long long gc = c0 + j;
+            Bs[l][kk][j] = (gk < p.k && gc < p.n) ? p.B[((long long)gk * p.ldb + gc) * L + l] : 0.0;
+        }
+        __syncthreads();
+        const int kmax = min(TK, p.k - k0);
+        for (int kk = 0; kk < kmax; ++kk) {
+            double a[L], b[L], q[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                a[l] = As[l][ty][kk];
+                b[l] = Bs[l][kk][tx];
+            }
+            md_mul<L, TP_FMA>(a, b, q);
+            md_add<L>(acc, q, acc);
+        }
+        __syncthreads();
+    }
+    const long long r = r0 + ty, c = c0 + tx;
+    if (r < p.m && c < p.n) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) p.C[(r * p.ldc + c) * L + l] = acc[l];
+    }
+}
+
+// ------------------------------------------------------------------ plan --
+static long long rup(long long x, long long q) { return (x + q - 1) / q * q; }
+
+struct Stage {
+    int kind;      // 0 init, 1 scan rows, 2 scan cols, 3 finish, 4 slice A, 5 slice B, 6 gemm, 7 recon, 8 vector
+    int L;         // limb count of the stage's problems (0: any)
+    long long first, count;   // range in the entry table
+};
+
+struct Plan {
+    std::vector<BProb> probs;
+    std::vector<BEnt> ents;
+    std::vector<Stage> stages;
+    BProb *dprobs = nullptr;
+    BEnt *dents = nullptr;
+    void *arena = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Plan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        (void)hipFree(dprobs);
+        (void)hipFree(dents);
+        (void)hipFree(arena);
+    }
+};
+
+static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, int target_bits, int spread)
+{
+    const bool i8 = (method == MDG_METHOD_INT8_OZAKI);
+    // arena layout
+    size_t bytes = 0;
+    auto take = [&](size_t b) {
+        size_t off = bytes;
+        bytes += (b + 255) / 256 * 256;
+        return off;
+    };
+    struct Offs {
+        size_t aall, ball, d, e, f, cs;
+    };
+    std::vector<Offs> offs(count);
+    pl.probs.resize(count);
+    for (int i = 0; i < count; ++i) {
+        const mdg_problem &q = pr[i];
+        BProb &b = pl.probs[i];
+        std::memset(&b, 0, sizeof(b));
+        b.L = q.L;
+        b.m = q.m;
+        b.n = q.n;
+        b.k = q.k;
+        b.lda = q.lda;
+        b.ldb = q.ldb;
+        b.ldc = q.ldc;
+        b.A = q.A;
+        b.B = q.B;
+        b.C = q.C;
+        if (!i8 || q.m == 0 || q.n == 0 || q.k == 0) continue;
+        b.w = 8;
+        b.S = plan_int8_slices(q.L, target_bits > 0 ? target_bits : 53 * q.L, spread, spread);
+        MDG_REQUIRE(b.S > 0 && b.S * 8 <= 32 * (q.L == 2 ? 8 : q.L == 4 ? 12 : q.L == 8 ? 20 : 36) - 1,
+                    MDG_ERROR_RANGE, "problem %d: slice plan does not fit", i);
+        b.kp = (int)rup(q.k, 128);
+        b.mp = (int)rup(q.m, 128);
+        b.np = (int)rup(q.n, 128);
+        b.kmax = (b.kp <= 131071) ? (131071 / b.kp) * b.kp : 130944;
+        b.nplanes = 0;
+        for (int g = 0; g < b.S; ++g) b.nplanes += ((g + 1) * b.kp + b.kmax - 1) / b.kmax;
+        offs[i].aall = take((size_t)b.mp * b.S * b.kp);
+        offs[i].ball = take((size_t)b.np * b.S * b.kp);
+        offs[i].d = take((size_t)b.nplanes * b.m * b.n * 4);
+        offs[i].e = take((size_t)b.m * 4);
+        offs[i].f = take((size_t)b.n * 4);
+        offs[i].cs = take((size_t)b.n * 8);
+    }
+    if (bytes) {
+        MDG_HIP_CHECK(hipMalloc(&pl.arena, bytes));
+        char *base = (char *)pl.arena;
+        for (int i = 0; i < count; ++i) {
+            BProb &b = pl.probs[i];
+            if (!b.S) continue;
+            b.Aall = (int8_t *)(base + offs[i].aall);
+            b.Ball = (int8_t *)(base + offs[i].ball);
+            b.D = (int *)(base + offs[i].d);
+            b.E = (int *)(base + offs[i].e);
+            b.F = (int *)(base + offs[i].f);
+            b.colsum = (double *)(base + offs[i].cs);
+        }
+    }
+    // entry tables, one stage per (kind, L)
+    auto add_stage = [&](int kind, int L, auto gen) {
+        Stage st{kind, L, (long long)pl.ents.size(), 0};
+        for (int i = 0; i < count; ++i) {
+            const BProb &b = pl.probs[i];
+            if (L && b.L != L) continue;
+            if (b.m == 0 || b.n == 0 || b.k == 0) continue;
+            if (i8 && !b.S) continue;
+            gen(i, b);
+        }
+        st.count = (long long)pl.ents.size() - st.first;
+        if (st.count) pl.stages.push_back(st);
+    };
+    auto push = [&](int i, int x, int y, int z) { pl.ents.push_back(BEnt{i, x, y, z}); };
+    const int Ls[4] = {2, 4, 8, 16};
+    if (i8) {
+        add_stage(0, 0, [&](int i, const BProb &b) {
+            for (int x = 0; x < (b.n + 255) / 256; ++x) push(i, x, 0, 0);
+        });
+        for (int L : Ls)
+            add_stage(1, L, [&](int i, const BProb &b) {
+                for (int x = 0; x < (b.m + 3) / 4; ++x) push(i, x, 0, 0);
+            });
+        for (int L : Ls)
+            add_stage(2, L, [&](int i, const BProb &b) {
+                for (int x = 0; x < (b.n + 63) / 64; ++x)
+                    for (int y = 0; y < (b.k + 255) / 256; ++y) push(i, x, y, 0);
+            });
+        add_stage(3, 0, [&](int i, const BProb &b) {
+            for (int x = 0; x < (b.n + 255) / 256; ++x) push(i, x, 0, 0);
+        });
+        for (int L : Ls)
+            add_stage(4, L, [&](int i, const BProb &b) {
+                const long long nb = ((long long)b.mp * b.kp + 255) / 256;
+                for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
+            });
+        for (int L : Ls)
+            add_stage(5, L, [&](int i, const BProb &b) {
+                const long long nb = ((long long)b.np * b.kp + 255) / 256;
+                for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
+            });
+        // GEMM tiles: deepest planes of every problem first, so long blocks start early
+        {
+            Stage st{6, 0, (long long)pl.ents.size(), 0};
+            int maxp = 0;
+            for (const BProb &b : pl.probs) maxp = std::max(maxp, b.nplanes);
+            for (int z = maxp - 1; z >= 0; --z)
+                for (int i = 0; i < count; ++i) {
+                    const BProb &b = pl.probs[i];
+                    if (!b.S || z >= b.nplanes || b.m == 0 || b.n == 0 || b.k == 0) continue;
+                    for (int x = 0; x < b.mp / 128; ++x)
+                        for (int y = 0; y < b.np / 128; ++y) push(i, x, y, z);
+                }
+            st.count = (long long)pl.ents.size() - st.first;
+            if (st.count) pl.stages.push_back(st);
+        }
+        for (int L : Ls)
+            add_stage(7, L, [&](int i, const BProb &b) {
+                const long long nb = ((long long)b.m * b.n + 255) / 256;
+                for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
+            });
+    } else {
+        for (int L : Ls)
+            add_stage(8, L, [&](int i, const BProb &b) {
+                for (int x = 0; x < (b.m + 15) / 16; ++x)
+                    for (int y = 0; y < (b.n + 15) / 16; ++y) push(i, x, y, 0);
+            });
+    }
+    for (const Stage &st : pl.stages)
+        MDG_REQUIRE(st.count <= 2147483647ll, MDG_ERROR_NOT_SUPPORTED, "batch too large for one launch");
+    MDG_HIP_CHECK(hipMalloc(&pl.dprobs, sizeof(BProb) * std::max(1, count)));
+    MDG_HIP_CHECK(hipMemcpy(pl.dprobs, pl.probs.data(), sizeof(BProb) * count, hipMemcpyHostToDevice));
+    MDG_HIP_CHECK(hipMalloc(&pl.dents, sizeof(BEnt) * std::max<size_t>(1, pl.ents.size())));
+    MDG_HIP_CHECK(
+        hipMemcpy(pl.dents, pl.ents.data(), sizeof(BEnt) * pl.ents.size(), hipMemcpyHostToDevice));
+    return MDG_SUCCESS;
+}
+
+static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
+{
+    const dim3 grid((unsigned)st.count), blk(256);
+    const BEnt *T = pl.dents + st.first;
+    const BProb *P = pl.dprobs;
+#define MDG_BY_L(kern)                                                                      \
+    switch (st.L) {                                                                         \
+    case 2: hipLaunchKernelGGL(kern<2>, grid, blk, 0, s, P, T); break;                      \
+    case 4: hipLaunchKernelGGL(kern<4>, grid, blk, 0, s, P, T); break;                      \
+    case 8: hipLaunchKernelGGL(kern<8>, grid, blk, 0, s, P, T); break;                      \
+    default: hipLaunchKernelGGL(kern<16>, grid, blk, 0, s, P, T); break;                    \
+    }
+    switch (st.kind) {
+    case 0: hipLaunchKernelGGL(b_init_kernel, grid, blk, 0, s, P, T); break;
+    case 1: MDG_BY_L(b_scan_rows_kernel); break;
+    case 2: MDG_BY_L(b_scan_cols_kernel); break;
+    case 3: hipLaunchKernelGGL(b_finish_cols_kernel, grid, blk, 0, s, P, T); break;
+    case 4: MDG_BY_L(b_slice_a_kernel); break;
+    case 5: MDG_BY_L(b_slice_b_kernel); break;
+    case 6: hipLaunchKernelGGL(b_diag_i8_kernel, grid, blk, 0, s, P, T); break;
+    case 7: MDG_BY_L(b_reconstruct_kernel); break;
+    default: MDG_BY_L(b_vector_kernel); break;
+    }
+#undef MDG_BY_L
+    MDG_LAUNCH_CHECK("batched stage");
+    return MDG_SUCCESS;
+}
+
+static int run_plan(const Plan &pl, hipStream_t s)
+{
+    for (const Stage &st : pl.stages) {
+        int rc = launch_stage(pl, st, s);
+        if (rc) return rc;
+    }
+    return MDG_SUCCESS;
+}
 
 static int run_loop(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s)
 {
@@ -37,12 +570,8 @@ static int run_loop(int count, const mdg_problem *pr, int method, int target_bit
     return MDG_SUCCESS;
 }
 
-struct GraphEntry {
-    hipGraphExec_t exec = nullptr;
-};
-
-static std::mutex g_graph_mu;
-static std::map<std::string, GraphEntry> g_graphs;
+static std::mutex g_plan_mu;
+static std::map<std::string, Plan *> g_plans;
 
 static std::string signature(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s)
 {
@@ -60,35 +589,56 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
                  hipStream_t s)
 {
     const int spread = spread_bits < 0 ? 16 : spread_bits;
+    // zero-k problems: C = 0 (vector path writes zeros for k == 0 via the loop)
     if (!persistent) return run_loop(count, pr, method, target_bits, spread, s);
+    for (int i = 0; i < count; ++i)
+        MDG_REQUIRE(pr[i].k > 0 || pr[i].m == 0 || pr[i].n == 0, MDG_ERROR_NOT_SUPPORTED,
+                    "persistent batches need k > 0 (problem %d)", i);
+    MDG_REQUIRE(s != nullptr, MDG_ERROR_NOT_SUPPORTED, "persistent batches need a non-null stream");
 
     const std::string key = signature(count, pr, method, target_bits, spread, s);
-    std::lock_guard<std::mutex> lock(g_graph_mu);
-    auto it = g_graphs.find(key);
-    if (it == g_graphs.end()) {
-        // one uncaptured pass sizes every scratch buffer (capture must not allocate)
-        int rc = run_loop(count, pr, method, target_bits, spread, s);
-        if (rc) return rc;
-        MDG_REQUIRE(s != nullptr, MDG_ERROR_NOT_SUPPORTED, "graph replay needs a non-null stream");
-        MDG_HIP_CHECK(hipStreamSynchronize(s));
-        hipGraph_t graph = nullptr;
-        MDG_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        rc = run_loop(count, pr, method, target_bits, spread, s);
-        hipError_t ce = hipStreamEndCapture(s, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        MDG_HIP_CHECK(ce);
-        GraphEntry e;
-        hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        MDG_HIP_CHECK(ie);
-        g_graphs[key] = e;
-        return MDG_SUCCESS;   // the uncaptured pass already computed this call
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+        MDG_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
+        return MDG_SUCCESS;
     }
-    MDG_HIP_CHECK(hipGraphLaunch(it->second.exec, s));
-    return MDG_SUCCESS;
+    Plan *pl = new Plan();
+    const bool fused = (method == MDG_METHOD_INT8_OZAKI || method == MDG_METHOD_VECTOR);
+    int rc = fused ? build_plan(*pl, count, pr, method, target_bits, spread) : MDG_SUCCESS;
+    if (rc) {
+        delete pl;
+        return rc;
+    }
+    // one uncaptured pass sizes every library scratch buffer (capture must not allocate)
+    rc = fused ? run_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
+    if (rc) {
+        delete pl;
+        return rc;
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    hipGraph_t graph = nullptr;
+    if (e == hipSuccess) e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete pl;
+        MDG_HIP_CHECK(e);
+    }
+    rc = fused ? run_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
+    e = hipStreamEndCapture(s, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        delete pl;
+        if (rc) return rc;
+        MDG_HIP_CHECK(e);
+    }
+    e = hipGraphInstantiate(&pl->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        delete pl;
+        MDG_HIP_CHECK(e);
+    }
+    g_plans[key] = pl;
+    return MDG_SUCCESS;   // the uncaptured pass already computed this call
 }
 
 }  // namespace mdg

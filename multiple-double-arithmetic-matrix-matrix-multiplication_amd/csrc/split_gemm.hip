@@ -810,6 +810,19 @@ static int plan_slices(Plan &p, int target_bits)
     return MDG_SUCCESS;
 }
 
+// slice count of the INT8 path for a given precision target and spreads
+// (the batched pipeline sizes its slices with this, without a read-back)
+int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b)
+{
+    Plan p{};
+    p.method = MDG_METHOD_INT8_OZAKI;
+    p.L = L;
+    p.spread_a = spread_a;
+    p.spread_b = spread_b;
+    p.kp = 128;
+    return plan_slices(p, target_bits) ? -1 : p.S;
+}
+
 template <int L>
 static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const double *A, long long lda,
                           const double *B, long long ldb, double *C, long long ldc, hipStream_t s,

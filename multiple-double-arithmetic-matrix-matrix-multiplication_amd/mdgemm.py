@@ -162,8 +162,24 @@ def gemm_batched(problems, method="int8", target_bits=0, spread_bits=-1, persist
                          p["B"].data_ptr(), p.get("ldb", p["n"]), p["C"].data_ptr(),
                          p.get("ldc", p["n"]))
     meth = METHODS[method] if isinstance(method, str) else method
+    if stream is None and persistent:
+        # graph capture needs a non-null stream: run on a side stream ordered
+        # after, and before, the caller's current stream
+        import torch
+        global _side_stream
+        if _side_stream is None:
+            _side_stream = torch.cuda.Stream()
+        cur = torch.cuda.current_stream()
+        _side_stream.wait_stream(cur)
+        _check(lib().mdg_gemm_batched(len(problems), arr, meth, target_bits, spread_bits, persistent,
+                                      ctypes.c_void_p(_side_stream.cuda_stream)))
+        cur.wait_stream(_side_stream)
+        return
     _check(lib().mdg_gemm_batched(len(problems), arr, meth, target_bits, spread_bits, persistent,
                                   _stream(stream)))
+
+
+_side_stream = None
 
 
 def split4pd(x, p, stream=None):

@@ -1,0 +1,86 @@
+"""Batched GEMMs of uneven shape (config 4's mix of double- and quad-double
+problems): the fused persistent pipeline with hipGraph replay against the
+per-problem loop and the oracle."""
+import numpy as np
+import pytest
+
+from conftest import assert_bits_equal, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(mdg, orc, count, seed, lo=16, hi=160):
+    rng = np.random.default_rng(seed)
+    probs, host = [], []
+    for i in range(count):
+        L = 2 if i % 2 == 0 else 4
+        m, n, k = (int(v) for v in rng.integers(lo, hi + 1, 3))
+        A = orc.generate(orc.GEN_QD, L, m, k, seed * 1000 + 2 * i, -8, 8)
+        B = orc.generate(orc.GEN_QD, L, k, n, seed * 1000 + 2 * i + 1, -8, 8)
+        probs.append({"L": L, "m": m, "n": n, "k": k, "A": to_dev(A), "B": to_dev(B),
+                      "C": to_dev(np.full(m * n * L, np.nan))})
+        host.append((L, m, n, k, A, B))
+    return probs, host
+
+
+def _refill(probs, host, orc, seed):
+    # new data in the same device buffers (a graph replay reads them afresh)
+    for i, (p, (L, m, n, k, _, _)) in enumerate(zip(probs, host)):
+        A = orc.generate(orc.GEN_QD, L, m, k, seed + 2 * i, -8, 8)
+        B = orc.generate(orc.GEN_QD, L, k, n, seed + 2 * i + 1, -8, 8)
+        p["A"].copy_(to_dev(A))
+        p["B"].copy_(to_dev(B))
+        host[i] = (L, m, n, k, A, B)
+
+
+def test_batched_vector_bit_identical(mdg, orc, torch_cuda):
+    probs, host = _batch(mdg, orc, 24, 3)
+    for persistent in (0, 1, 1):   # loop, graph build, graph replay
+        for p in probs:
+            p["C"].fill_(np.nan)
+        mdg.gemm_batched(probs, method="vector", persistent=persistent)
+        torch_cuda.cuda.synchronize()
+        for p, (L, m, n, k, A, B) in zip(probs, host):
+            assert_bits_equal(to_host(p["C"]), orc.gemm(L, m, n, k, A, B, mode=orc.TP_FMA),
+                              f"vector batch persistent={persistent} L={L} {m}x{n}x{k}")
+
+
+def test_batched_int8_persistent_equals_loop(mdg, orc, torch_cuda):
+    probs, host = _batch(mdg, orc, 24, 5)
+    mdg.gemm_batched(probs, method="int8", persistent=0, spread_bits=16)
+    torch_cuda.cuda.synchronize()
+    loop = [to_host(p["C"]).copy() for p in probs]
+    for p, (L, m, n, k, A, B), C in zip(probs, host, loop):
+        err, _ = orc.normwise_error(L, m, n, k, A, B, C)
+        assert err <= 4.0 * k * 2.0 ** (-53 * L), (L, m, n, k, err)
+    for rep in range(3):           # build + capture, then two graph replays
+        for p in probs:
+            p["C"].fill_(np.nan)
+        mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        for i, (p, C) in enumerate(zip(probs, loop)):
+            assert_bits_equal(to_host(p["C"]), C, f"int8 batch rep={rep} problem {i}")
+
+
+def test_batched_int8_graph_replay_sees_new_data(mdg, orc, torch_cuda):
+    probs, host = _batch(mdg, orc, 12, 7)
+    mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=16)   # build
+    for rep in range(3):
+        _refill(probs, host, orc, 9000 + 100 * rep)
+        torch_cuda.cuda.synchronize()
+        mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=16)   # replay
+        torch_cuda.cuda.synchronize()
+        for p, (L, m, n, k, A, B) in zip(probs, host):
+            err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(p["C"]))
+            assert err <= 4.0 * k * 2.0 ** (-53 * L), (rep, L, m, n, k, err)
+
+
+@pytest.mark.parametrize("method", ["f64", "bf16"])
+def test_batched_loop_methods(mdg, orc, torch_cuda, method):
+    probs, host = _batch(mdg, orc, 8, 11)
+    for persistent in (0, 1, 1):
+        mdg.gemm_batched(probs, method=method, persistent=persistent, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        for p, (L, m, n, k, A, B) in zip(probs, host):
+            err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(p["C"]))
+            assert err <= 4.0 * k * 2.0 ** (-53 * L), (method, persistent, L, m, n, k, err)
