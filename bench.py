@@ -146,12 +146,25 @@ def main():
         work = P * 2.0 * m * n * k
         peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
         kernel = "diag_f64_kernel"
+    # launches of the dominant kernel per step: one per INT32 plane (INT8),
+    # one diagonal-GEMM launch covering all diagonals otherwise
+    if head_path == "int8":
+        kp = -(-k // 128) * 128
+        kmax = (131071 // kp) * kp if kp <= 131071 else 130944
+        launches = sum(-(-((g + 1) * kp) // kmax) for g in range(head["slices"]))
+        if head["engine"] == "mfma":
+            launches = 1
+    else:
+        launches = 1
     achieved = work / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
-    workload = f"qd{n}"
+    workload = f"qd{n}" if L == 4 else f"L{L}n{n}"
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": unit,
                 "frac": (achieved / peak) if achieved else None,
                 "traffic": read_traffic(kernel, workload), "kernel": kernel,
-                "work_per_launch": work, "avg_launch_ms": gemm_ms}
+                "launches_per_step": launches, "work_per_launch": work / launches,
+                "avg_launch_ms": gemm_ms / launches,
+                "note": "achieved = algorithmic ops (P*2mnk) / diagonal-GEMM time from HIP events "
+                        "on the launch stream; INT8 ops count as 2 per multiply-add"}
 
     cpu = None
     if not a.skip_cpu:
