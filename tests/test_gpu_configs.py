@@ -1,0 +1,147 @@
+"""The five workload configurations at their full sizes, checked through
+size-independent properties: the exact checker's normwise error on sampled
+entries (<= c k 2^(-53 L), c = 4), bit-identity of the vector GEMM with the
+oracle on sampled rows, agreement between paths, and batched == loop.
+Timings are printed (pytest -s) for the record."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import assert_bits_equal, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _bound(L, k):
+    return 4.0 * k * 2.0 ** (-53 * L)
+
+
+def _timed(torch, fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) * 1e3
+
+
+def _err(orc, L, n, k, A, B, C, nsamp, seed=3):
+    return orc.normwise_error(L, n, n, k, A, B, C, nsamp=nsamp, seed=seed)[0]
+
+
+def test_config1_dd_n128(mdg, orc, torch_cuda):
+    # double-double n = 128, hi ~ U[-1,1], lo = hi 2^-53 U[-.5,.5] renormalised
+    L, n = 2, 128
+    A = mdg.fill_random(mdg.GEN_DD, L, n, n, 101)
+    B = mdg.fill_random(mdg.GEN_DD, L, n, n, 102)
+    Ah, Bh = to_host(A), to_host(B)
+    assert_bits_equal(Ah, orc.generate(orc.GEN_DD, L, n, n, 101), "generator")
+    want = orc.gemm(L, n, n, n, Ah, Bh, mode=orc.TP_DEKKER)   # the reference's CPU path
+    got = to_host(mdg.vector_gemm(L, n, n, n, A, B, twoprod=mdg.TWOPROD_DEKKER))
+    assert_bits_equal(got, want, "dd vector GEMM vs host GEMM")
+    assert_bits_equal(to_host(mdg.vector_gemm(L, n, n, n, A, B)), want, "fma two_prod")
+    for method in ("int8", "f64", "bf16"):
+        C, info = mdg.gemm(L, n, n, n, A, B, method=method)
+        e = _err(orc, L, n, n, Ah, Bh, to_host(C), n * n)
+        print(f"cfg1 dd n=128 {method}: err {e:.3e} S={info['slices']}")
+        assert e <= _bound(L, n)
+
+
+def test_config2_qd_n1024_f64_split_vs_vector(mdg, orc, torch_cuda):
+    L, n = 4, 1024
+    A = mdg.fill_random(mdg.GEN_QD, L, n, n, 201, -8, 8)
+    B = mdg.fill_random(mdg.GEN_QD, L, n, n, 202, -8, 8)
+    Ah, Bh = to_host(A), to_host(B)
+    Cv, tv = _timed(torch_cuda, lambda: mdg.vector_gemm(L, n, n, n, A, B))
+    (Cs, info), ts = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method="f64"))
+    Cv, Cs = to_host(Cv), to_host(Cs)
+    # vector GEMM bit-identical to the host loops on 3 sampled rows
+    for r in (0, 517, 1023):
+        want = orc.gemm_rows(L, r, r + 1, n, n, Ah, Bh, mode=orc.TP_FMA, nthreads=8)
+        assert_bits_equal(Cv[r * n * L:(r + 1) * n * L], want, f"row {r}")
+    ev = _err(orc, L, n, n, Ah, Bh, Cv, 1024)
+    es = _err(orc, L, n, n, Ah, Bh, Cs, 1024)
+    print(f"cfg2 qd n=1024: vector {tv:.1f} ms err {ev:.3e}; f64 split {ts:.1f} ms err {es:.3e} "
+          f"S={info['slices']} w={info['slice_bits']}")
+    assert ev <= _bound(L, n) and es <= _bound(L, n)
+    assert es <= ev   # the split path is at least as accurate as the reference arithmetic
+
+
+def test_config3_od_2048_int8_bf16_vs_f64(mdg, orc, torch_cuda):
+    L, n = 8, 2048
+    A = mdg.fill_random(mdg.GEN_QD, L, n, n, 301, -30, 30)
+    B = mdg.fill_random(mdg.GEN_QD, L, n, n, 302, -30, 30)
+    Ah, Bh = to_host(A), to_host(B)
+    res = {}
+    for method in ("f64", "int8", "bf16"):
+        (C, info), t = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method=method))
+        e = _err(orc, L, n, n, Ah, Bh, to_host(C), 4096)
+        res[method] = to_host(C)
+        print(f"cfg3 od n=2048 {method}: {t:.1f} ms err {e:.3e} S={info['slices']} "
+              f"w={info['slice_bits']} P={info['pair_gemms']}")
+        assert e <= _bound(L, n)
+    # INT8 and BF16 slice identically and sum exactly: identical bits
+    assert_bits_equal(res["int8"], res["bf16"], "int8 vs bf16")
+
+
+def test_config4_batched_1024_mixed(mdg, orc, torch_cuda):
+    rng = np.random.default_rng(404)
+    probs = []
+    for i in range(1024):
+        L = 2 if i < 512 else 4
+        m, n, k = (int(v) for v in rng.integers(16, 513, 3))
+        probs.append({"L": L, "m": m, "n": n, "k": k,
+                      "A": mdg.fill_random(mdg.GEN_QD, L, m, k, 40000 + 2 * i, -8, 8),
+                      "B": mdg.fill_random(mdg.GEN_QD, L, k, n, 40001 + 2 * i, -8, 8),
+                      "C": mdg.empty(m * n * L)})
+    eff = sum(2.0 * p["m"] * p["n"] * p["k"] for p in probs)
+    _, t_loop = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8", persistent=0,
+                                                            spread_bits=16))
+    loop = [to_host(p["C"]).copy() for p in probs[::64]]
+    _, t_build = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8", persistent=1,
+                                                             spread_bits=16))
+    _, t_replay = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8", persistent=1,
+                                                              spread_bits=16))
+    print(f"cfg4 1024 problems ({eff:.3e} eff. flops): loop {t_loop:.1f} ms, graph build "
+          f"{t_build:.1f} ms, graph replay {t_replay:.1f} ms ({eff / t_replay / 1e9:.1f} GFLOP/s)")
+    for p, C in zip(probs[::64], loop):
+        assert_bits_equal(to_host(p["C"]), C, "persistent vs loop")
+        L, m, n, k = p["L"], p["m"], p["n"], p["k"]
+        e = orc.normwise_error(L, m, n, k, to_host(p["A"]), to_host(p["B"]), C, nsamp=256)[0]
+        assert e <= _bound(L, k)
+
+
+@pytest.fixture(scope="module")
+def hd4096(mdg, torch_cuda):
+    L, n = 16, 4096
+    A = mdg.fill_random(mdg.GEN_QD, L, n, n, 501, -60, 60)
+    B = mdg.fill_random(mdg.GEN_QD, L, n, n, 502, -60, 60)
+    return L, n, A, B, to_host(A), to_host(B)
+
+
+@pytest.mark.parametrize("method", ["int8", "f64"])
+def test_config5_hd_4096_split(mdg, orc, torch_cuda, hd4096, method):
+    L, n, A, B, Ah, Bh = hd4096
+    (C, info), t = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method=method))
+    Ch = to_host(C)
+    del C
+    e = _err(orc, L, n, n, Ah, Bh, Ch, 256)
+    print(f"cfg5 hd n=4096 {method}: {t:.1f} ms ({2.0 * n ** 3 / t / 1e9:.1f} GFLOP/s eff) err {e:.3e} "
+          f"S={info['slices']} w={info['slice_bits']} P={info['pair_gemms']}")
+    assert e <= _bound(L, n)
+    mdg.lib().mdg_release_workspace()
+
+
+def test_config5_hd_4096_vector(mdg, orc, torch_cuda, hd4096):
+    L, n, A, B, Ah, Bh = hd4096
+    C, t = _timed(torch_cuda, lambda: mdg.vector_gemm(L, n, n, n, A, B))
+    Ch = to_host(C)
+    e = _err(orc, L, n, n, Ah, Bh, Ch, 256)
+    print(f"cfg5 hd n=4096 vector: {t:.1f} ms ({2.0 * n ** 3 / t / 1e9:.1f} GFLOP/s eff) err {e:.3e}")
+    assert e <= _bound(L, n)
+    # 16 entries of row 1234 against the oracle's host loops
+    r, c0, w = 1234, 2000, 16
+    Arow = Ah[r * n * L:(r + 1) * n * L]
+    Bsub = np.ascontiguousarray(Bh.reshape(n, n, L)[:, c0:c0 + w, :]).ravel()
+    want = orc.gemm(L, 1, w, n, Arow, Bsub, mode=orc.TP_FMA)
+    assert_bits_equal(Ch.reshape(n, n, L)[r, c0:c0 + w, :].ravel(), want, "hd row 1234")
