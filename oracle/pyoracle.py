@@ -269,7 +269,15 @@ def normwise_error(L, m, n, k, A, B, C, nsamp=4096, seed=12345, nthreads=None):
         idx = rng.choice(m * n, size=nsamp, replace=False)
         rr, cc = np.divmod(idx, n)
     err, den, _ = exact_error(L, m, n, k, A, B, C, rr, cc, nthreads)
-    return float(np.sqrt(np.sum(err ** 2)) / np.sqrt(np.sum(den ** 2))), int(rr.size)
+    return float(_norm2(err) / _norm2(den)), int(rr.size)
+
+
+def _norm2(v):
+    """2-norm without underflow of the squares (hexa-double errors are ~2^-800)."""
+    s = float(np.max(np.abs(v))) if v.size else 0.0
+    if s == 0.0:
+        return 0.0
+    return s * float(np.sqrt(np.sum((v / s) ** 2)))
 
 
 def exact_entry(L, k, A, lda, B, ldb, r, c):

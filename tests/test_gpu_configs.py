@@ -103,7 +103,7 @@ def test_config4_batched_1024_mixed(mdg, orc, torch_cuda):
     _, t_replay = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8", persistent=1,
                                                               spread_bits=16))
     print(f"cfg4 1024 problems ({eff:.3e} eff. flops): loop {t_loop:.1f} ms, graph build "
-          f"{t_build:.1f} ms, graph replay {t_replay:.1f} ms ({eff / t_replay / 1e9:.1f} GFLOP/s)")
+          f"{t_build:.1f} ms, graph replay {t_replay:.1f} ms ({eff / t_replay / 1e6:.1f} GFLOP/s)")
     for p, C in zip(probs[::64], loop):
         assert_bits_equal(to_host(p["C"]), C, "persistent vs loop")
         L, m, n, k = p["L"], p["m"], p["n"], p["k"]
@@ -126,7 +126,7 @@ def test_config5_hd_4096_split(mdg, orc, torch_cuda, hd4096, method):
     Ch = to_host(C)
     del C
     e = _err(orc, L, n, n, Ah, Bh, Ch, 256)
-    print(f"cfg5 hd n=4096 {method}: {t:.1f} ms ({2.0 * n ** 3 / t / 1e9:.1f} GFLOP/s eff) err {e:.3e} "
+    print(f"cfg5 hd n=4096 {method}: {t:.1f} ms ({2.0 * n ** 3 / t / 1e6:.1f} GFLOP/s eff) err {e:.3e} "
           f"S={info['slices']} w={info['slice_bits']} P={info['pair_gemms']}")
     assert e <= _bound(L, n)
     mdg.lib().mdg_release_workspace()
@@ -137,7 +137,7 @@ def test_config5_hd_4096_vector(mdg, orc, torch_cuda, hd4096):
     C, t = _timed(torch_cuda, lambda: mdg.vector_gemm(L, n, n, n, A, B))
     Ch = to_host(C)
     e = _err(orc, L, n, n, Ah, Bh, Ch, 256)
-    print(f"cfg5 hd n=4096 vector: {t:.1f} ms ({2.0 * n ** 3 / t / 1e9:.1f} GFLOP/s eff) err {e:.3e}")
+    print(f"cfg5 hd n=4096 vector: {t:.1f} ms ({2.0 * n ** 3 / t / 1e6:.1f} GFLOP/s eff) err {e:.3e}")
     assert e <= _bound(L, n)
     # 16 entries of row 1234 against the oracle's host loops
     r, c0, w = 1234, 2000, 16

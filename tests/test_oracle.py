@@ -157,3 +157,15 @@ def test_split_parts_sum_to_stored_value(orc):
                 want = sum(Fraction(float(t)) for t in v)
                 got = sum(Fraction(float(t)) for t in per_entry[i])
                 assert got == want
+
+
+def test_normwise_error_does_not_underflow(orc):
+    # hexa-double-sized errors (2^-800 absolute) must not vanish when squared
+    L, m, n, k = 16, 2, 2, 3
+    A = orc.generate(orc.GEN_QD, L, m, k, 61, 0, 0)
+    B = orc.generate(orc.GEN_QD, L, k, n, 62, 0, 0)
+    C = orc.gemm(L, m, n, k, A, B)
+    C2 = C.copy()
+    C2[L - 1] += 2.0 ** -800
+    e1, _ = orc.normwise_error(L, m, n, k, A, B, C2)
+    assert e1 > 0.0
