@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--paths", default="int8,int8_mfma,f64,bf16,vector",
-                    help="int8 = INT8 slices with rocBLAS diagonal GEMMs, int8_mfma = with this "
-                         "library's INT8 MFMA kernel")
+                    help="int8 = INT8 slices with the autotuned diagonal-GEMM engine, int8_mfma = "
+                         "forced to this library's INT8 MFMA kernel")
     return ap.parse_args()
 
 

@@ -371,6 +371,8 @@ struct Plan {
     BEnt *dents = nullptr;
     void *arena = nullptr;
     hipGraphExec_t exec = nullptr;
+    Stage gemm_alt{6, 0, 0, 0};   // problem-major order of the GEMM tiles
+    bool use_alt = false;         // chosen by timing both orders once
     ~Plan()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -496,6 +498,17 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 }
             st.count = (long long)pl.ents.size() - st.first;
             if (st.count) pl.stages.push_back(st);
+            // the alternative schedule the autotuner tries: problem-major
+            Stage alt{6, 0, (long long)pl.ents.size(), 0};
+            for (int i = 0; i < count; ++i) {
+                const BProb &b = pl.probs[i];
+                if (!b.S || b.m == 0 || b.n == 0 || b.k == 0) continue;
+                for (int z = b.nplanes - 1; z >= 0; --z)
+                    for (int x = 0; x < b.mp / 128; ++x)
+                        for (int y = 0; y < b.np / 128; ++y) push(i, x, y, z);
+            }
+            alt.count = (long long)pl.ents.size() - alt.first;
+            pl.gemm_alt = alt;
         }
         for (int L : Ls)
             add_stage(7, L, [&](int i, const BProb &b) {
@@ -550,9 +563,33 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
 static int run_plan(const Plan &pl, hipStream_t s)
 {
     for (const Stage &st : pl.stages) {
-        int rc = launch_stage(pl, st, s);
+        int rc = launch_stage(pl, (st.kind == 6 && pl.use_alt) ? pl.gemm_alt : st, s);
         if (rc) return rc;
     }
+    return MDG_SUCCESS;
+}
+
+// Autotune the batch schedule: run the pipeline with each GEMM tile order and
+// keep the faster (both produce identical results).
+static int tune_plan(Plan &pl, hipStream_t s)
+{
+    if (pl.gemm_alt.count == 0) return run_plan(pl, s);
+    float ms[2] = {0.f, 0.f};
+    for (int o = 0; o < 2; ++o) {
+        pl.use_alt = (o == 1);
+        hipEvent_t e0, e1;
+        MDG_HIP_CHECK(hipEventCreate(&e0));
+        MDG_HIP_CHECK(hipEventCreate(&e1));
+        MDG_HIP_CHECK(hipEventRecord(e0, s));
+        int rc = run_plan(pl, s);
+        MDG_HIP_CHECK(hipEventRecord(e1, s));
+        MDG_HIP_CHECK(hipEventSynchronize(e1));
+        MDG_HIP_CHECK(hipEventElapsedTime(&ms[o], e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+    }
+    pl.use_alt = ms[1] < ms[0];
     return MDG_SUCCESS;
 }
 
@@ -610,8 +647,9 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         delete pl;
         return rc;
     }
-    // one uncaptured pass sizes every library scratch buffer (capture must not allocate)
-    rc = fused ? run_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
+    // uncaptured passes size every library scratch buffer (capture must not
+    // allocate) and, for the fused pipeline, choose the GEMM tile order
+    rc = fused ? tune_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
     if (rc) {
         delete pl;
         return rc;

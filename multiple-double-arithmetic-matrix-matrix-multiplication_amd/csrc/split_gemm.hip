@@ -22,12 +22,14 @@
 // blocks of the reference's Toeplitz operand, never materialised here.
 //
 // Three GEMM back ends share the slicing:
-//   INT8  w = 8   v_mfma_i32_16x16x64_i8   exact INT32 sums, widened to
-//                                          INT64 in-loop when (g+1)kp*2^14
-//                                          could reach 2^31
+//   INT8  w = 8   INT8 x INT8 -> INT32     exact while the depth K of one
+//                 (rocBLAS gemm_ex, or     plane has K * 2^14 < 2^31; deeper
+//                 v_mfma_i32_16x16x64_i8)  diagonals are cut into K-chunk
+//                                          planes added in INT64 afterwards;
+//                                          the engine is autotuned per shape
 //   BF16  w = 8   v_mfma_f32_16x16x32_bf16 exact FP32 sums (< 2^24), widened
 //                                          to INT64 every 1024 k
-//   FP64  w <= 26 v_mfma_f64_16x16x4_f64   exact FP64 sums (< 2^53), widened
+//   FP64  w <= 25 v_mfma_f64_16x16x4_f64   exact FP64 sums (< 2^53), widened
 //                                          to INT64 every 2^(55-2w) k
 #include <climits>
 #include <cmath>
@@ -37,6 +39,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "common.hpp"
 #include "md_arith.hpp"
@@ -718,8 +721,37 @@ static BlasHandle *blas_handle()
 
 static int int8_backend_choice()
 {
-    // 0 auto (rocBLAS), 1 rocBLAS, 2 hand-written MFMA kernel
+    // 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel
     return int8_backend();
+}
+
+// Autotuner cache of the INT8 engine per shape bucket: limb count, slice
+// count and the power-of-two buckets of m, n and the padded depth.
+struct TuneKey {
+    int L, S, m, n, kp;
+};
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;
+
+static std::tuple<int, int, int, int, int, int> bucket(const TuneKey &k)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    auto lg = [](int v) { return v <= 1 ? 0 : 32 - __builtin_clz((unsigned)(v - 1)); };
+    return {dev, k.L, k.S, lg(k.m), lg(k.n), lg(k.kp)};
+}
+
+static int tuned_engine(const TuneKey &k)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(bucket(k));
+    return it == g_tune.end() ? 0 : it->second;
+}
+
+static void remember_engine(const TuneKey &k, int engine)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[bucket(k)] = engine;
 }
 
 static int i8_planes_rocblas(const int8_t *Aall, const int8_t *Ball, int S, int kp, int kmax, int m, int n, int *D,
@@ -896,18 +928,48 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
         tm.mark(2);
-        const int be = int8_backend_choice();
-        p.backend = (be == 2) ? 2 : 1;
-        if (p.backend == 1) {
-            rc = i8_planes_rocblas(Aall, Ball, S, kp, kmax, m, n, D, s);
-            if (rc) return rc;
-        } else {
+        auto run_engine = [&](int engine) -> int {
+            if (engine == 1) return i8_planes_rocblas(Aall, Ball, S, kp, kmax, m, n, D, s);
             MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
             const int tiles_m8 = mp8 / 256, tiles_n = np / 128;
             dim3 grid(tiles_m8 * tiles_n, 1, (unsigned)nplanes);
             hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(512), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
                                tiles_n, D);
             MDG_LAUNCH_CHECK("diag_i8_kernel");
+            return MDG_SUCCESS;
+        };
+        const int be = int8_backend_choice();
+        bool planes_done = false;
+        if (be == 1 || be == 2) {
+            p.backend = be;
+        } else {
+            // autotuned: the first call of a shape bucket times both engines on
+            // the same slices (they produce identical planes) and keeps the faster
+            const TuneKey key{L, S, m, n, kp};
+            p.backend = tuned_engine(key);
+            if (p.backend == 0) {
+                float ms[3] = {0.f, 0.f, 0.f};
+                for (int engine = 1; engine <= 2; ++engine) {
+                    hipEvent_t e0, e1;
+                    MDG_HIP_CHECK(hipEventCreate(&e0));
+                    MDG_HIP_CHECK(hipEventCreate(&e1));
+                    MDG_HIP_CHECK(hipEventRecord(e0, s));
+                    rc = run_engine(engine);
+                    MDG_HIP_CHECK(hipEventRecord(e1, s));
+                    MDG_HIP_CHECK(hipEventSynchronize(e1));
+                    MDG_HIP_CHECK(hipEventElapsedTime(&ms[engine], e0, e1));
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    if (rc) return rc;
+                }
+                p.backend = (ms[2] < ms[1]) ? 2 : 1;
+                remember_engine(key, p.backend);
+                planes_done = true;   // both engines wrote the same exact planes
+            }
+        }
+        if (!planes_done) {
+            rc = run_engine(p.backend);
+            if (rc) return rc;
         }
         tm.mark(3);
         hipLaunchKernelGGL((reconstruct_i8_kernel<L>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, kp,

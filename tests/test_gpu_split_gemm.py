@@ -8,14 +8,16 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_mfma", "bf16"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "bf16"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
-    """int8 runs on rocBLAS (the default engine), int8_mfma on this library's
-    INT8 MFMA kernel; both must give the same exact diagonal sums."""
-    if method == "int8_mfma":
-        mdg.set_int8_backend(2)
+    """int8 runs the autotuned engine, int8_rocblas rocBLAS gemm_ex and
+    int8_mfma this library's INT8 MFMA kernel; all give the same exact
+    diagonal sums."""
+    engine = {"int8_rocblas": 1, "int8_mfma": 2}.get(method)
+    if engine:
+        mdg.set_int8_backend(engine)
         try:
             return mdg.gemm(L, m, n, k, A, B, method="int8", **kw)
         finally:
@@ -102,7 +104,7 @@ def test_split_gemm_rejects_non_finite(mdg, torch_cuda):
     assert ei.value.status == 5
 
 
-@pytest.mark.parametrize("method", ["int8", "int8_mfma"])
+@pytest.mark.parametrize("method", ["int8_rocblas", "int8_mfma"])
 def test_int8_chunked_diagonals(mdg, orc, torch_cuda, method):
     # S * kp > 2^17: the deepest diagonals are cut into K-chunks whose INT32
     # planes are added in INT64 by the reconstruction
@@ -122,7 +124,13 @@ def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
     m, n, k = 300, 260, 333
     A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 17, -20, 20))
     B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 18, -20, 20))
-    C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8")
+    C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8_rocblas")
     C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
     assert i1["backend"] == 1 and i2["backend"] == 2
     assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
+    # the autotuner times both engines on the first call of a shape bucket,
+    # then keeps its choice; every call gives the same bits
+    for _ in range(2):
+        C3, i3 = gemm(mdg, L, m, n, k, A, B, "int8")
+        assert i3["backend"] in (1, 2)
+        assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
