@@ -40,7 +40,10 @@ typedef enum mdg_method {
     MDG_METHOD_VECTOR = 0,      /* one thread per output entry, p-double mul+add */
     MDG_METHOD_F64_SPLIT = 1,   /* signed fixed-point slices on FP64 MFMA */
     MDG_METHOD_INT8_OZAKI = 2,  /* 8-bit slices on INT8 MFMA, INT32 accumulate */
-    MDG_METHOD_BF16_OZAKI = 3   /* 8-bit slices on BF16 MFMA, FP32 accumulate */
+    MDG_METHOD_BF16_OZAKI = 3,  /* 8-bit slices on BF16 MFMA, FP32 accumulate */
+    MDG_METHOD_INT8_RESIDUE = 4 /* INT8 residues modulo coprime moduli on INT8
+                                   MFMA, exact CRT reconstruction (no
+                                   counterpart in the reference) */
 } mdg_method;
 
 /* two_prod flavour of the vector path */
@@ -52,11 +55,15 @@ typedef enum mdg_twoprod {
 /* What a split-product GEMM did (filled when a non-NULL pointer is given). */
 typedef struct mdg_gemm_info {
     int method;            /* mdg_method actually run */
-    int slices;            /* S: slices per operand entry */
-    int slice_bits;        /* w: bits per slice */
-    int diagonals;         /* slice-sum diagonals kept (== slices) */
-    long long pair_gemms;  /* P: slice-pair products issued, S(S+1)/2 */
-    int total_bits;        /* W = S * w fixed-point bits per entry */
+    int slices;            /* S: slices per operand entry (residue method:
+                              moduli) */
+    int slice_bits;        /* w: bits per slice (residue method: 8) */
+    int diagonals;         /* slice-sum diagonals kept (== slices; residue
+                              method: wide moduli, three GEMMs each) */
+    long long pair_gemms;  /* P: slice-pair products issued, S(S+1)/2 (residue
+                              method: INT8 GEMMs of depth k, wide moduli
+                              counted 4 times) */
+    int total_bits;        /* W fixed-point bits per entry (S * w for slices) */
     int spread_a;          /* exponent spread estimate of A, bits */
     int spread_b;          /* exponent spread estimate of B, bits */
     int flushed;           /* 1 if accumulators were widened inside the K loop */
@@ -116,8 +123,12 @@ int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda,
  * Split-product GEMM C = A * B (no reference counterpart as a single call;
  * it replaces the reference's split -> tensor-core GEMM -> squeeze chain,
  * C++/TCtest.cu:53-114, for signed inputs of any exponent range).
- *   method       MDG_METHOD_F64_SPLIT / _INT8_OZAKI / _BF16_OZAKI, or
- *                MDG_METHOD_VECTOR (then identical to mdg_vector_gemm, FMA)
+ *   method       MDG_METHOD_F64_SPLIT / _INT8_OZAKI / _BF16_OZAKI /
+ *                _INT8_RESIDUE, or MDG_METHOD_VECTOR (then identical to
+ *                mdg_vector_gemm, FMA).  _INT8_RESIDUE computes the same
+ *                fixed-point product from residues modulo pairwise coprime
+ *                moduli (Ozaki scheme II): INT8 GEMMs linear, not quadratic,
+ *                in the bit count; k <= 2^22.
  *   target_bits  fixed-point precision goal; <= 0 means 53*L.  The slice
  *                count is chosen from it and from the exponent spread of
  *                each operand's rows/columns (a small device->host read).

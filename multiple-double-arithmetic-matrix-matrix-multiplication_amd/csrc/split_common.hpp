@@ -1,0 +1,33 @@
+// split_common.hpp -- pieces shared by the split-product GEMMs: the slice
+// GEMMs (split_gemm.hip) and the residue GEMM (residue_gemm.hip).
+#pragma once
+#include <rocblas/rocblas.h>
+
+#include <mutex>
+
+#include "common.hpp"
+
+namespace mdg {
+
+constexpr int ZERO_SCALE = -4000;   // scale exponent of an all-zero row/column
+
+// one rocBLAS handle per device, created (and its kernels loaded) on first use
+struct BlasHandle {
+    rocblas_handle h = nullptr;
+    std::mutex mu;
+};
+BlasHandle *blas_handle();
+
+// Scale exponents of A's rows (*E, m ints) and B's columns (*F, n ints) in
+// scratch slots 0 and 1: every |entry| < 2^E, all-zero rows/columns get
+// ZERO_SCALE.  With fixed_spread < 0 the exponent spreads are measured and
+// read back (and non-finite limbs rejected); otherwise both spreads are
+// fixed_spread and the stream stays asynchronous.
+int scan_scales(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                int fixed_spread, hipStream_t s, int **E, int **F, int *spread_a, int *spread_b);
+
+inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
+
+inline unsigned grid_for(long long nel, int bs) { return (unsigned)((nel + bs - 1) / bs); }
+
+}  // namespace mdg

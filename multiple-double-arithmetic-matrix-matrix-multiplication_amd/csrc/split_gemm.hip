@@ -41,9 +41,9 @@
 #include <mutex>
 #include <tuple>
 
-#include "common.hpp"
 #include "md_arith.hpp"
 #include "slices.hpp"
+#include "split_common.hpp"
 
 namespace mdg {
 
@@ -51,8 +51,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-constexpr int ZERO_SCALE = -4000;   // scale exponent of an all-zero row/column
 
 // ------------------------------------------------------------------ scan --
 // stats[0] (int): max spread over rows/cols, stats[1] (int): non-finite flag
@@ -696,14 +694,10 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
 // The diagonal GEMMs are plain INT8 x INT8 -> INT32 GEMMs; rocBLAS runs each
 // K-chunk plane as one gemm_ex call (column-major view: D_p^T (n x m) =
 // Bt_chunk^T-op (n x K) * A_chunk (K x m), lda = ldb = S*kp).
-struct BlasHandle {
-    rocblas_handle h = nullptr;
-    std::mutex mu;
-};
 static std::mutex g_blas_mu;
 static std::map<int, BlasHandle *> g_blas;
 
-static BlasHandle *blas_handle()
+BlasHandle *blas_handle()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
@@ -789,8 +783,6 @@ static int i8_planes_rocblas(const int8_t *Aall, const int8_t *Ball, int S, int 
 }
 
 // ---------------------------------------------------------------- driver --
-static inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
-
 template <typename K>
 static inline unsigned blocks_for(long long nel, int bs) { return (unsigned)((nel + bs - 1) / bs); }
 
@@ -857,21 +849,16 @@ int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b)
 }
 
 template <int L>
-static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const double *A, long long lda,
-                          const double *B, long long ldb, double *C, long long ldc, hipStream_t s,
-                          mdg_gemm_info *info)
+static int scan_scales_l(int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                         int fixed_spread, hipStream_t s, int **Eo, int **Fo, int *spread_a, int *spread_b)
 {
-    const int m = p.m, n = p.n, k = p.k;
-    // scratch: E (m), F (n), colsum (n), stats (4)
+    // scratch: E (m), F (n), stats (8) in slot 0, colsum (n) in slot 1
     int *E = (int *)scratch(s, sizeof(int) * (size_t)(m + n + 8), 0);
     MDG_REQUIRE(E, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     int *F = E + m;
     int *stats = F + n;
     double *colsum = (double *)scratch(s, sizeof(double) * (size_t)n, 1);
     MDG_REQUIRE(colsum, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-
-    StageTimer tm(s, profiling_on());
-    tm.mark(0);
     hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, F, (long long)n, INT_MIN);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, stats, 8ll, 0);
     hipLaunchKernelGGL(fill_double_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, colsum, (long long)n, 0.0);
@@ -881,24 +868,53 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                        ldb, rpb, F, colsum, stats);
     hipLaunchKernelGGL(finish_cols_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, k, n, F, colsum, stats);
     MDG_LAUNCH_CHECK("scan kernels");
-    tm.mark(1);
     if (fixed_spread < 0) {
         int hstats[4];
         MDG_HIP_CHECK(hipMemcpyAsync(hstats, stats, sizeof(hstats), hipMemcpyDeviceToHost, s));
         MDG_HIP_CHECK(hipStreamSynchronize(s));
         MDG_REQUIRE(hstats[1] == 0, MDG_ERROR_NON_FINITE, "input contains Inf or NaN limbs");
-        p.spread_a = hstats[0];
-        p.spread_b = hstats[2];
+        *spread_a = hstats[0];
+        *spread_b = hstats[2];
     } else {
         // caller-supplied allowance: no read-back, the stream stays asynchronous
-        p.spread_a = fixed_spread;
-        p.spread_b = fixed_spread;
+        *spread_a = fixed_spread;
+        *spread_b = fixed_spread;
     }
+    *Eo = E;
+    *Fo = F;
+    return MDG_SUCCESS;
+}
+
+int scan_scales(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                int fixed_spread, hipStream_t s, int **E, int **F, int *spread_a, int *spread_b)
+{
+    switch (L) {
+    case 2: return scan_scales_l<2>(m, n, k, A, lda, B, ldb, fixed_spread, s, E, F, spread_a, spread_b);
+    case 4: return scan_scales_l<4>(m, n, k, A, lda, B, ldb, fixed_spread, s, E, F, spread_a, spread_b);
+    case 8: return scan_scales_l<8>(m, n, k, A, lda, B, ldb, fixed_spread, s, E, F, spread_a, spread_b);
+    case 16: return scan_scales_l<16>(m, n, k, A, lda, B, ldb, fixed_spread, s, E, F, spread_a, spread_b);
+    }
+    set_error("unsupported limb count L=%d", L);
+    return MDG_ERROR_INVALID_VALUE;
+}
+
+template <int L>
+static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const double *A, long long lda,
+                          const double *B, long long ldb, double *C, long long ldc, hipStream_t s,
+                          mdg_gemm_info *info)
+{
+    const int m = p.m, n = p.n, k = p.k;
+    StageTimer tm(s, profiling_on());
+    tm.mark(0);
+    int *E = nullptr, *F = nullptr;
+    int rc = scan_scales_l<L>(m, n, k, A, lda, B, ldb, fixed_spread, s, &E, &F, &p.spread_a, &p.spread_b);
+    if (rc) return rc;
+    tm.mark(1);
 
     p.kp = (int)round_up(k, 128);
     p.mp = (int)round_up(m, 128);
     p.np = (int)round_up(n, 128);
-    int rc = plan_slices(p, target_bits);
+    rc = plan_slices(p, target_bits);
     if (rc) {
         set_error("no slice width fits k=%d at %d target bits", k, target_bits);
         return rc;
@@ -1040,9 +1056,14 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     return MDG_SUCCESS;
 }
 
+int residue_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                 double *C, long long ldc, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s);
+
 int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
                long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s)
 {
+    if (method == MDG_METHOD_INT8_RESIDUE)
+        return residue_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
     Plan p{};
     p.method = method;
     p.L = L;

@@ -1,6 +1,7 @@
-"""Split-product GEMMs (FP64 / INT8 / BF16 MFMA) against the exact checker:
-normwise error || C_exact - C || / || |A||B| || within c*k*2^(-53L), and
-agreement with the vector GEMM (the reference's arithmetic)."""
+"""Split-product GEMMs (FP64 / INT8 / BF16 MFMA slices, INT8 residues)
+against the exact checker: normwise error || C_exact - C || / || |A||B| ||
+within c*k*2^(-53L), and agreement with the vector GEMM (the reference's
+arithmetic)."""
 import numpy as np
 import pytest
 
@@ -8,7 +9,7 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "bf16"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "bf16", "int8_residue"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
@@ -41,7 +42,12 @@ def test_split_gemm_error_bound(mdg, orc, torch_cuda, method, L):
         C = to_host(C)
         err, _ = orc.normwise_error(L, m, n, k, A, B, C)
         assert err <= _bound(L, k), f"{method} L={L}: err={err:.3e} bound={_bound(L, k):.3e} {info}"
-        assert info["slices"] >= 1 and info["pair_gemms"] == info["slices"] * (info["slices"] + 1) // 2
+        S = info["slices"]
+        if method == "int8_residue":
+            # narrow moduli: one INT8 GEMM each, wide moduli: four
+            assert info["pair_gemms"] == (S - info["diagonals"]) + 4 * info["diagonals"]
+        else:
+            assert S >= 1 and info["pair_gemms"] == S * (S + 1) // 2
         V = orc.gemm(L, m, n, k, A, B, mode=orc.TP_FMA)
         errv, _ = orc.normwise_error(L, m, n, k, A, B, V)
         assert errv <= _bound(L, k)
@@ -134,3 +140,16 @@ def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
         C3, i3 = gemm(mdg, L, m, n, k, A, B, "int8")
         assert i3["backend"] in (1, 2)
         assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
+
+
+def test_residue_chunked_depth(mdg, orc, torch_cuda):
+    # 2 kp > 130944: the depth-2k GEMMs of the wide moduli are cut into
+    # K-chunks whose INT32 planes are added in INT64 before the reduction
+    L, m, n, k = 4, 16, 24, 70001
+    A = orc.generate(orc.GEN_QD, L, m, k, 15, -4, 4)
+    B = orc.generate(orc.GEN_QD, L, k, n, 16, -4, 4)
+    C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method="int8_residue")
+    assert info["flushed"] == 1 and info["diagonals"] > 0, info
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=96)
+    assert err <= _bound(L, k), (err, info)
+
