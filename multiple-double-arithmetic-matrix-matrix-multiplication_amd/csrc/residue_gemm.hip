@@ -42,21 +42,28 @@
 
 namespace mdg {
 
-constexpr int RES_P8 = 72;       // 16-bit halfwords of the widest fixed-point integer (36 words)
-constexpr int RES_PB = 144;      // bytes of it
+constexpr int RES_NC = 32;       // 37-bit chunks of the widest fixed-point integer (36 words)
 constexpr int RES_KC = 130944;   // GEMM depth with exact INT32 sums: RES_KC * 2^14 < 2^31, multiple of 128
+constexpr int RES_TK = 64;       // B residue tile depth (k rows)
 
 // Device view of one modulus set (n = n8 + n16 moduli, narrow ones first).
+// All tables hold exact integers as doubles: the kernels evaluate them in
+// FP64 FMA chains whose partial sums stay below 2^53, hence exact.
 struct ResTab {
-    int n8, n16;
-    int nwc;                    // words per CRT constant (incl. a sign word), a multiple of 4
-    const double *md;           // m_i
-    const double *inv;          // 1 / m_i
-    const double *frac;         // y_i / m_i, y_i = (M/m_i)^-1 mod m_i
-    const uint32_t *cw;         // c_i = (M/m_i) y_i, [n][nwc] 32-bit words, least significant first
-    const uint32_t *Mw;         // M, [nwc] words
-    const uint32_t *p8;         // 2^(16h) mod m_i, [n8][RES_P8]
-    const int *pb;              // balanced 2^(8b) mod m_j, [n16][RES_PB]
+    int n8, n16;                // narrow / wide moduli
+    int ng8;                    // narrow groups: triples of narrow moduli (the last may be shorter)
+    int nw;                     // 24-bit words per CRT constant (those of M; Z keeps its sign in the carry)
+    const double *md;           // m_i                                                  [n]
+    const double *inv;          // 1 / m_i                                              [n]
+    const double *pw37;         // balanced 2^(37c) mod m_i                              [n8][RES_NC]
+    const double *pw37h;        // wide m_j: balanced 2^(37c) mod m_j = 256 hi + lo, hi   [n16][RES_NC]
+    const double *pw37l;        //                                                lo     [n16][RES_NC]
+    const double *gP;           // group moduli: narrow triple products P_g, then the wide moduli   [G]
+    const double *ginv;         // 1 / P_g                                              [G]
+    const double *gfrac;        // y_g / P_g, y_g = (M/P_g)^-1 mod P_g                  [G]
+    const double *gE;           // narrow group g: E_x = (P/m_x)((P/m_x)^-1 mod m_x) of its members, 0 if absent [ng8][3]
+    const double *cw;           // c_g = (M/P_g) y_g, [G][nw] 24-bit words, least significant first
+    const double *Mw;           // M, [nw] 24-bit words
 };
 
 // v mod m in [0, m) for an integer-valued double |v| < 2^53: the FP64
@@ -70,133 +77,227 @@ __device__ __forceinline__ double mod_pos(double v, double md, double inv)
     return r;
 }
 
-// Balanced residues of |N| (sign applied afterwards) for every modulus of t.
-// emit8(i, r): narrow modulus i, r in [-128, 127];
-// emit16(j, hi, lo): wide modulus j, r = 256 hi + lo, hi, lo in [-128, 127].
-template <int NW, typename E8, typename E16>
-__device__ __forceinline__ void residues(uint32_t (&N)[NW], const ResTab &t, E8 emit8, E16 emit16)
+// bits [37c, 37c + 37) of the non-negative NW-word integer N, as a double
+// (32-bit funnel shifts only: a 64-bit combination of two array words
+// would be turned into a 64-bit load and keep N in scratch memory)
+template <int NW>
+__device__ __forceinline__ double chunk37(const uint32_t (&N)[NW], int c)
 {
-    const bool neg = (int32_t)N[NW - 1] < 0;
+    const int pos = 37 * c, j = pos >> 5, off = pos & 31;
+    const uint32_t w0 = N[j], w1 = (j + 1 < NW) ? N[j + 1] : 0u, w2 = (j + 2 < NW) ? N[j + 2] : 0u;
+    const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, off);
+    const uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, off) & 31u;
+    return fma((double)hi, 4294967296.0, (double)lo);
+}
+
+template <int EPT> struct PackOf;
+template <> struct PackOf<4> { typedef uint32_t T; };
+template <> struct PackOf<2> { typedef uint16_t T; };
+template <> struct PackOf<1> { typedef uint8_t T; };
+
+// Balanced residue of an integer-valued double v, |v| < 2^51, modulo an odd
+// m: fl(v * fl(1/m)) is within |v|/m 2^-52 < 1/(2m) of v/m, and v/m is at
+// least 1/(2m) from every half-integer (m odd), so rint gives the nearest
+// quotient and the remainder fma is exact: r in [-(m-1)/2, (m-1)/2].
+__device__ __forceinline__ double mod_bal_odd(double v, double md, double inv)
+{
+    return fma(-rint(v * inv), md, v);
+}
+
+// |floor(x 2^shift)| as NC 37-bit chunks; returns the sign and the low byte
+// of the two's-complement value (its residue modulo 256)
+template <int L, int NWU, int NC>
+__device__ __forceinline__ double prep_entry(const double (&x)[L], int shift, double (&ch)[NC], uint32_t &low)
+{
+    uint32_t N[NWU];
+    to_fixed<L, NWU>(x, shift, N);
+    low = N[0] & 255u;
+    const bool neg = (int32_t)N[NWU - 1] < 0;
     if (neg) {
-        uint32_t c = 1;
+        uint32_t cy = 1;
 #pragma unroll
-        for (int j = 0; j < NW; ++j) {
-            const uint32_t v = ~N[j] + c;
-            c = (c && v == 0u) ? 1u : 0u;
+        for (int j = 0; j < NWU; ++j) {
+            const uint32_t v = ~N[j] + cy;
+            cy = (cy && v == 0u) ? 1u : 0u;
             N[j] = v;
         }
     }
-    for (int i = 0; i < t.n8; ++i) {
-        // halfword (16-bit) x (2^(16h) mod m < 2^8): sums below 2^31
-        const uint32_t *pw = t.p8 + i * RES_P8;
-        uint32_t acc = 0;
 #pragma unroll
-        for (int j = 0; j < NW; ++j) acc += (N[j] & 0xFFFFu) * pw[2 * j] + (N[j] >> 16) * pw[2 * j + 1];
-        const double md = t.md[i];
-        int r = (int)mod_pos((double)acc, md, t.inv[i]);
-        const int m = (int)md;
-        if (neg && r) r = m - r;
-        if (2 * r >= m) r -= m;
-        emit8(i, r);
+    for (int c = 0; c < NC; ++c) ch[c] = chunk37<NWU>(N, c);
+    return neg ? -1.0 : 1.0;
+}
+
+// Balanced residues of EPT entries that share one scale shift, packed EPT
+// bytes per plane:
+//   narrow plane i at o8 + i * plane8                       r in [-128, 127]
+//   wide modulus j at o16 + j * plane16: hi at +0, lo at +kp (and hi again
+//   at +2kp when triple), r = 256 hi + lo with hi, lo in [-128, 127].
+// |N| = |floor(x 2^shift)| is cut into at most 32 chunks of 37 bits; with
+// |2^(37c) mod m| (balanced) below 2^7 -- for a wide modulus split as
+// 256 hi + lo into two such tables -- every partial sum stays below 2^47,
+// exact in FP64 and inside mod_bal_odd's range.  Modulus 256 is the low
+// byte of N itself.
+template <int L, int NWU, int EPT>
+__device__ __forceinline__ void entry_residues(const double (&x)[EPT][L], int shift, const ResTab &t, int8_t *o8,
+                                               long long plane8, int8_t *o16, long long plane16, int kp, bool triple)
+{
+    constexpr int NC = (32 * NWU + 36) / 37;
+    typedef typename PackOf<EPT>::T PT;
+    double ch[EPT][NC];
+    double sg[EPT];
+    uint32_t low[EPT];
+    // entries prepared one by one in straight-line code (a loop over them
+    // would not be unrolled and would put the arrays in scratch memory)
+    sg[0] = prep_entry<L, NWU, NC>(x[0], shift, ch[0], low[0]);
+    if constexpr (EPT > 1) sg[1] = prep_entry<L, NWU, NC>(x[1], shift, ch[1], low[1]);
+    if constexpr (EPT > 2) {
+        sg[2] = prep_entry<L, NWU, NC>(x[2], shift, ch[2], low[2]);
+        sg[3] = prep_entry<L, NWU, NC>(x[3], shift, ch[3], low[3]);
+    }
+    int i = 0;
+    if (t.n8 > 0 && t.md[0] == 256.0) {
+        PT word = 0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) word |= (PT)(low[e] << (8 * e));
+        *(PT *)o8 = word;
+        i = 1;
+    }
+    for (; i < t.n8; ++i) {
+        const double *pw = t.pw37 + i * RES_NC;
+        const double md = t.md[i], inv = t.inv[i];
+        PT word = 0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            double v = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v = fma(ch[e][c], pw[c], v);
+            const int r = (int)mod_bal_odd(v * sg[e], md, inv);
+            word |= (PT)(((uint32_t)r & 255u) << (8 * e));
+        }
+        *(PT *)(o8 + i * plane8) = word;
     }
     for (int j = 0; j < t.n16; ++j) {
-        // byte x balanced (2^(8b) mod m, |.| < 2^15): |sum| < 2^31
-        const int *pw = t.pb + j * RES_PB;
-        int acc = 0;
+        const double *ph = t.pw37h + j * RES_NC;
+        const double *pl = t.pw37l + j * RES_NC;
+        const double md = t.md[t.n8 + j], inv = t.inv[t.n8 + j];
+        PT hw = 0, lw = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            acc += (int)(N[w] & 0xFFu) * pw[4 * w] + (int)((N[w] >> 8) & 0xFFu) * pw[4 * w + 1] +
-                   (int)((N[w] >> 16) & 0xFFu) * pw[4 * w + 2] + (int)(N[w] >> 24) * pw[4 * w + 3];
+        for (int e = 0; e < EPT; ++e) {
+            double vh = 0.0, vl = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                vh = fma(ch[e][c], ph[c], vh);
+                vl = fma(ch[e][c], pl[c], vl);
+            }
+            const double r1 = mod_bal_odd(vh * sg[e], md, inv);                  // |r1| < 2^15
+            const int r = (int)mod_bal_odd(fma(r1, 256.0, vl * sg[e]), md, inv);  // < 2^47 in
+            const int lo = ((r + 128) & 255) - 128;
+            const int hi = (r - lo) >> 8;
+            hw |= (PT)(((uint32_t)hi & 255u) << (8 * e));
+            lw |= (PT)(((uint32_t)lo & 255u) << (8 * e));
         }
-        const double md = t.md[t.n8 + j];
-        int r = (int)mod_pos((double)acc, md, t.inv[t.n8 + j]);
-        const int m = (int)md;
-        if (neg && r) r = m - r;
-        if (2 * r >= m) r -= m;
-        const int lo = ((r + 128) & 255) - 128;
-        emit16(j, (r - lo) >> 8, lo);
+        int8_t *o = o16 + j * plane16;
+        *(PT *)o = hw;
+        *(PT *)(o + kp) = lw;
+        if (triple) *(PT *)(o + 2 * kp) = hw;
     }
 }
 
-// A: thread per entry (r, kk) of the padded mp x kp domain, kk fastest.
-// Narrow planes R8[i][r][kk] (mp x kp), wide planes R16[j][r][2kp]: hi at
-// kk, lo at kp + kk.
+// entries per thread: packed stores of 4/2/1 bytes per plane, within the
+// register budget of the fixed-point words and chunks
+template <int L>
+struct ResEPT {
+    static constexpr int value = (L <= 4) ? 4 : (L == 8) ? 2 : 1;
+};
+
+// A: thread per EPT consecutive entries (r, kk..kk+EPT-1) of the padded
+// mp x kp domain.  Narrow planes R8[i][r][kk] (mp x kp), wide planes
+// R16[j][r][2kp]: hi at kk, lo at kp + kk.
 template <int L, int NWU>
 __global__ __launch_bounds__(256) void residue_a_kernel(int m, int k, int mp, int kp, const double *__restrict__ A,
                                                         long long lda, const int *__restrict__ E, int W, ResTab t,
                                                         int8_t *__restrict__ R8, int8_t *__restrict__ R16)
 {
+    constexpr int EPT = ResEPT<L>::value;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)mp * kp) return;
-    const long long r = idx / kp;
-    const int kk = (int)(idx - r * kp);
-    const long long plane = (long long)mp * kp;
-    int8_t *o8 = R8 + r * kp + kk;
-    int8_t *o16 = R16 + r * 2 * kp + kk;
-    if (r >= m || kk >= k || E[r] == ZERO_SCALE) {
-        for (int i = 0; i < t.n8; ++i) o8[i * plane] = 0;
-        for (int j = 0; j < t.n16; ++j) {
-            o16[2 * j * plane] = 0;
-            o16[2 * j * plane + kp] = 0;
-        }
-        return;
-    }
-    double x[L];
+    const int kq = kp / EPT;
+    if (idx >= (long long)mp * kq) return;
+    const long long r = idx / kq;
+    const int kk = (int)(idx - r * kq) * EPT;
+    double x[EPT][L];
+    int shift = 0;
+    const bool live = r < m && E[r] != ZERO_SCALE;
+    if (live) shift = W - 2 - E[r];
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = A[(r * lda + kk) * L + l];
-    uint32_t N[NWU];
-    to_fixed<L, NWU>(x, W - 2 - E[r], N);
-    residues<NWU>(
-        N, t, [&](int i, int v) { o8[i * plane] = (int8_t)v; },
-        [&](int j, int hi, int lo) {
-            o16[2 * j * plane] = (int8_t)hi;
-            o16[2 * j * plane + kp] = (int8_t)lo;
-        });
+    for (int e = 0; e < EPT; ++e)
+#pragma unroll
+        for (int l = 0; l < L; ++l) x[e][l] = (live && kk + e < k) ? A[(r * lda + kk + e) * L + l] : 0.0;
+    entry_residues<L, NWU, EPT>(x, shift, t, R8 + r * kp + kk, (long long)mp * kp, R16 + r * 2 * kp + kk,
+                                2ll * mp * kp, kp, false);
 }
 
-// B: thread per entry (kk, c) of the padded kp x np domain, kk fastest so
-// the transposed writes coalesce.  Narrow planes R8[i][c][kk] (np x kp),
-// wide planes R16[j][c][3kp]: hi at kk and 2kp + kk, lo at kp + kk, so that
-// [kp, 3kp) is (lo, hi) against A's (hi, lo).
+// B: a block stages a RES_TK x TC tile of B (rows kk, columns c; coalesced
+// along c) in LDS, then each thread takes EPT consecutive kk of one column,
+// so the transposed residue writes coalesce along kk.  Narrow planes
+// R8[i][c][kk] (np x kp), wide planes R16[j][c][3kp]: hi at kk and
+// 2kp + kk, lo at kp + kk -- [kp, 3kp) is (lo, hi) against A's (hi, lo).
 template <int L, int NWU>
 __global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, int np, const double *__restrict__ B,
                                                         long long ldb, const int *__restrict__ F, int W, ResTab t,
                                                         int8_t *__restrict__ R8, int8_t *__restrict__ R16)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)np * kp) return;
-    const long long c = idx / kp;
-    const int kk = (int)(idx - c * kp);
-    const long long plane = (long long)np * kp;
-    int8_t *o8 = R8 + c * kp + kk;
-    int8_t *o16 = R16 + c * 3 * kp + kk;
-    if (c >= n || kk >= k || F[c] == ZERO_SCALE) {
-        for (int i = 0; i < t.n8; ++i) o8[i * plane] = 0;
-        for (int j = 0; j < t.n16; ++j) {
-            o16[3 * j * plane] = 0;
-            o16[3 * j * plane + kp] = 0;
-            o16[3 * j * plane + 2 * kp] = 0;
-        }
-        return;
+    constexpr int EPT = ResEPT<L>::value;
+    constexpr int TPC = RES_TK / EPT;   // threads per column
+    constexpr int TC = 256 / TPC;       // columns per tile
+    constexpr int ROW = TC * L + 1;     // padded LDS row, doubles
+    __shared__ double tile[RES_TK * ROW];
+    const int ktiles = kp / RES_TK;
+    const int c0 = (int)(blockIdx.x / ktiles) * TC;
+    const int k0 = (int)(blockIdx.x % ktiles) * RES_TK;
+    for (int q = threadIdx.x; q < RES_TK * TC * L; q += 256) {
+        const int row = q / (TC * L), col = q - row * (TC * L);
+        const int kk = k0 + row, c = c0 + col / L;
+        tile[row * ROW + col] = (kk < k && c < n) ? B[((long long)kk * ldb + c0) * L + col] : 0.0;
     }
-    double x[L];
+    __syncthreads();
+    const int cc = threadIdx.x / TPC, kq = threadIdx.x % TPC;
+    const int c = c0 + cc, kk = k0 + kq * EPT;
+    const bool live = c < n && F[c] != ZERO_SCALE;
+    const int shift = live ? W - 2 - F[c] : 0;
+    double x[EPT][L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = B[((long long)kk * ldb + c) * L + l];
-    uint32_t N[NWU];
-    to_fixed<L, NWU>(x, W - 2 - F[c], N);
-    residues<NWU>(
-        N, t, [&](int i, int v) { o8[i * plane] = (int8_t)v; },
-        [&](int j, int hi, int lo) {
-            o16[3 * j * plane] = (int8_t)hi;
-            o16[3 * j * plane + kp] = (int8_t)lo;
-            o16[3 * j * plane + 2 * kp] = (int8_t)hi;
-        });
+    for (int e = 0; e < EPT; ++e)
+#pragma unroll
+        for (int l = 0; l < L; ++l) x[e][l] = live ? tile[(kq * EPT + e) * ROW + cc * L + l] : 0.0;
+    entry_residues<L, NWU, EPT>(x, shift, t, R8 + (long long)c * kp + kk, (long long)np * kp,
+                                R16 + (long long)c * 3 * kp + kk, 3ll * np * kp, kp, true);
 }
 
 // C[r][c] from the INT32 planes: narrow D8[i][q] (chunk q of nck1), wide
-// Dhh[j][q], Dll[j][q] (nck1 chunks) and Dx[j][q] (nck2 chunks).
-template <int L, int NWC>
+// Dhh[j][q], Dll[j][q] (nck1 chunks) and Dx[j][q] (nck2 chunks); plane p of a
+// block at p * ps.  Two-level CRT in exact FP64:
+//   * narrow triple g: R_g = (v_a E_a + v_b E_b + v_c E_c) mod P_g, P_g < 2^24
+//     (the plane values v enter directly while |v| E stays below 2^51,
+//     i.e. depth <= 8192 in one chunk; otherwise each v is reduced first);
+//   * wide modulus j: R = (hh 2^16 + xx 2^8 + ll) mod m_j;
+//   * Z = sum_g R_g c_g - q M in 24-bit word columns (R_g c_gw < 2^48, at
+//     most 17 narrow groups and 192 wide moduli: sums < 2^52.2, exact).
+// Z is folded into L limbs from its leading 48-bit piece: the first L + 1
+// pieces through fast_renorm, the rest through inc_d.
+constexpr int RES_GB = 4;   // narrow groups whose plane loads are issued together
+
+template <int NW>
+__device__ __forceinline__ void crt_acc(double (&acc)[NW], double &tq, const ResTab &t, int g, double R)
+{
+    tq = fma(R, t.gfrac[g], tq);
+    const double *cw = t.cw + (long long)g * NW;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) acc[w] = fma(R, cw[w], acc[w]);
+}
+
+template <int L, int NW>
 __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, int W, ResTab t, int nck1, int nck2,
+                                                                  long long ps, int reduce,
                                                                   const int *__restrict__ D8,
                                                                   const int *__restrict__ Dhh,
                                                                   const int *__restrict__ Dll,
@@ -213,66 +314,114 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
     for (int l = 0; l < L; ++l) y[l] = 0.0;
     const int er = E[r], fc = F[c];
     if (er != ZERO_SCALE && fc != ZERO_SCALE) {
-        const long long mn = (long long)m * n;
-        uint64_t acc[NWC];
+        double acc[NW];
 #pragma unroll
-        for (int w = 0; w < NWC; ++w) acc[w] = 0;
+        for (int w = 0; w < NW; ++w) acc[w] = 0.0;
         double tq = 0.0;
-        auto add = [&](int i, double v) {
-            const double res = mod_pos(v, t.md[i], t.inv[i]);
-            tq = fma(res, t.frac[i], tq);
-            const uint32_t ri = (uint32_t)res;
-            const uint32_t *cw = t.cw + (long long)i * NWC;
+        for (int g0 = 0; g0 < t.ng8; g0 += RES_GB) {
+            int v[RES_GB][3];
 #pragma unroll
-            for (int w = 0; w < NWC; ++w) acc[w] += (uint64_t)ri * cw[w];
-        };
-        for (int i = 0; i < t.n8; ++i) {
-            long long v = 0;
-            for (int q = 0; q < nck1; ++q) v += D8[((long long)i * nck1 + q) * mn + idx];
-            add(i, (double)v);
-        }
-        for (int j = 0; j < t.n16; ++j) {
-            long long hh = 0, ll = 0, xx = 0;
-            for (int q = 0; q < nck1; ++q) {
-                hh += Dhh[((long long)j * nck1 + q) * mn + idx];
-                ll += Dll[((long long)j * nck1 + q) * mn + idx];
+            for (int u = 0; u < RES_GB; ++u)
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int i = 3 * (g0 + u) + x;
+                    v[u][x] = (i < t.n8) ? D8[(long long)i * nck1 * ps + idx] : 0;
+                }
+#pragma unroll
+            for (int u = 0; u < RES_GB; ++u) {
+                const int g = g0 + u;
+                if (g >= t.ng8) break;
+                const double *Eg = t.gE + 3 * g;
+                double V;
+                if (!reduce) {
+                    V = fma((double)v[u][0], Eg[0], fma((double)v[u][1], Eg[1], (double)v[u][2] * Eg[2]));
+                } else {
+                    V = 0.0;
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const int i = 3 * g + x;
+                        if (i >= t.n8) break;
+                        long long sum = v[u][x];
+                        for (int q = 1; q < nck1; ++q) sum += D8[((long long)i * nck1 + q) * ps + idx];
+                        V = fma(mod_pos((double)sum, t.md[i], t.inv[i]), Eg[x], V);
+                    }
+                }
+                crt_acc<NW>(acc, tq, t, g, mod_pos(V, t.gP[g], t.ginv[g]));
             }
-            for (int q = 0; q < nck2; ++q) xx += Dx[((long long)j * nck2 + q) * mn + idx];
-            add(t.n8 + j, (double)(hh * 65536 + xx * 256 + ll));
         }
-        // Z = sum r_i c_i - q M, carried into 32-bit two's-complement words
-        const long long q = (long long)rint(tq);
-        uint32_t z[NWC];
+        for (int j0 = 0; j0 < t.n16; j0 += RES_GB) {
+            int hh[RES_GB], ll[RES_GB], xx[RES_GB];
+#pragma unroll
+            for (int u = 0; u < RES_GB; ++u) {
+                const bool on = j0 + u < t.n16;
+                const long long j = j0 + u;
+                hh[u] = on ? Dhh[j * nck1 * ps + idx] : 0;
+                ll[u] = on ? Dll[j * nck1 * ps + idx] : 0;
+                xx[u] = on ? Dx[j * nck2 * ps + idx] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < RES_GB; ++u) {
+                const int j = j0 + u;
+                if (j >= t.n16) break;
+                long long h = hh[u], l = ll[u], x = xx[u];
+                for (int q = 1; q < nck1; ++q) {
+                    h += Dhh[((long long)j * nck1 + q) * ps + idx];
+                    l += Dll[((long long)j * nck1 + q) * ps + idx];
+                }
+                for (int q = 1; q < nck2; ++q) x += Dx[((long long)j * nck2 + q) * ps + idx];
+                const int g = t.ng8 + j;   // h 2^16 + x 2^8 + l: exact, < 2^53
+                crt_acc<NW>(acc, tq, t, g,
+                            mod_pos(fma((double)h, 65536.0, fma((double)x, 256.0, (double)l)), t.gP[g], t.ginv[g]));
+            }
+        }
+        // Z = sum R_g c_g - q M in two's-complement 24-bit words
+        const double q = rint(tq);
+        long long z[NW];
         long long carry = 0;
 #pragma unroll
-        for (int w = 0; w < NWC; ++w) {
-            const long long v = (long long)acc[w] - q * (long long)t.Mw[w] + carry;
-            z[w] = (uint32_t)v;
-            carry = v >> 32;
+        for (int w = 0; w < NW; ++w) {
+            const long long v = (long long)fma(-q, t.Mw[w], acc[w]) + carry;
+            z[w] = v & 0xFFFFFF;
+            carry = v >> 24;
         }
-        const bool neg = (int32_t)z[NWC - 1] < 0;
+        const bool neg = carry < 0;
         if (neg) {
-            uint32_t cc = 1;
+            long long cy = 1;
 #pragma unroll
-            for (int w = 0; w < NWC; ++w) {
-                const uint32_t v = ~z[w] + cc;
-                cc = (cc && v == 0u) ? 1u : 0u;
-                z[w] = v;
+            for (int w = 0; w < NW; ++w) {
+                const long long v = (0xFFFFFF - z[w]) + cy;
+                z[w] = v & 0xFFFFFF;
+                cy = v >> 24;
             }
         }
-        // fold |Z| 2^sc from the top word down; words more than L*53 + 64
-        // bits below the leading one are under the last limb's rounding
-        const int sc = er + fc + 4 - 2 * W;
-        constexpr int KEEP = (53 * L + 63) / 32 + 2;
-        int left = KEEP;
+        // 48-bit pieces, the leading one first: K pieces carry at least
+        // 1 + 48 (K - 1) >= 53 L + 20 bits of |Z|
+        constexpr int NP = NW / 2;
+        constexpr int K = (53 * L + 19) / 48 + 2;
+        double d[NP];
+        int tp = -1;
 #pragma unroll
-        for (int w = NWC - 1; w >= 0; --w) {
-            if (z[w] != 0u && left > 0) md_inc_d<L>(y, ldexp((double)z[w], 32 * w + sc));
-            if (z[w] != 0u || left < KEEP) --left;
+        for (int p = 0; p < NP; ++p) {
+            d[p] = fma((double)z[2 * p + 1], 16777216.0, (double)z[2 * p]);
+            tp = (d[p] != 0.0) ? p : tp;
         }
-        if (neg) {
+        if (tp >= 0) {
+            const int sc = er + fc + 4 - 2 * W;
+            double piece[K];
 #pragma unroll
-            for (int l = 0; l < L; ++l) y[l] = -y[l];
+            for (int s = 0; s < K; ++s) {
+                double v = 0.0;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) v = (p == tp - s) ? d[p] : v;
+                piece[s] = ldexp(v, 48 * (tp - s) + sc);
+            }
+            fast_renorm<L>(piece, y);
+#pragma unroll
+            for (int s = L + 1; s < K; ++s) md_inc_d<L>(y, piece[s]);
+            if (neg) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) y[l] = -y[l];
+            }
         }
     }
     double *o = C + (r * ldc + c) * L;
@@ -418,69 +567,100 @@ struct ResTabHost {
 static std::mutex g_res_mu;
 static std::map<std::tuple<int, int, int, int>, ResTabHost *> g_res;
 
-// device tables of a modulus set with CRT constants of nwc words (cached)
-static ResTabHost *res_table(int n8, int n16, int nwc)
+static double balanced(long long p, long long m) { return (double)(2 * p >= m ? p - m : p); }
+
+// value of a big integer as 24-bit words (false if it needs more than nw)
+static bool big_words24(const Big &b, int nw, double *out)
+{
+    const int bits = big_bitlen(b);
+    if ((bits + 23) / 24 > nw) return false;
+    for (int w = 0; w * 24 < bits; ++w) {
+        uint32_t v = 0;
+        for (int bit = 0; bit < 24; ++bit) {
+            const int pos = 24 * w + bit;
+            if ((pos >> 5) < (int)b.size() && ((b[pos >> 5] >> (pos & 31)) & 1u)) v |= 1u << bit;
+        }
+        out[w] = (double)v;
+    }
+    return true;
+}
+
+// device tables of a modulus set with CRT constants of nw 24-bit words (cached)
+static ResTabHost *res_table(int n8, int n16, int nw)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(g_res_mu);
-    const auto key = std::make_tuple(dev, n8, n16, nwc);
+    const auto key = std::make_tuple(dev, n8, n16, nw);
     auto it = g_res.find(key);
     if (it != g_res.end()) return it->second;
 
     const std::vector<int> mods = moduli_of(n8, n16);
     const int n = (int)mods.size();
-    std::vector<double> md(n), inv(n), frac(n);
-    std::vector<uint32_t> cw((size_t)n * nwc, 0u), Mw(nwc, 0u), p8((size_t)std::max(n8, 1) * RES_P8, 0u);
-    std::vector<int> pb((size_t)std::max(n16, 1) * RES_PB, 0);
-    Big M{1u};
-    for (int m : mods) big_mul_small(M, (uint32_t)m);
-    if ((int)M.size() > nwc - 1) return nullptr;
-    std::copy(M.begin(), M.end(), Mw.begin());
+    const int ng8 = (n8 + 2) / 3, G = ng8 + n16;
+    // the groups: narrow triples (3g, 3g+1, 3g+2), then each wide modulus
+    std::vector<long long> P(G);
+    std::vector<std::vector<int>> members(G);
+    for (int i = 0; i < n8; ++i) members[i / 3].push_back(i);
+    for (int j = 0; j < n16; ++j) members[ng8 + j].push_back(n8 + j);
+    for (int g = 0; g < G; ++g) {
+        P[g] = 1;
+        for (int i : members[g]) P[g] *= mods[i];
+    }
+    const size_t o_inv = n, o_p37 = 2 * (size_t)n, o_p37h = o_p37 + (size_t)n8 * RES_NC,
+                 o_p37l = o_p37h + (size_t)n16 * RES_NC, o_gP = o_p37l + (size_t)n16 * RES_NC, o_ginv = o_gP + G,
+                 o_gfrac = o_ginv + G, o_gE = o_gfrac + G, o_cw = o_gE + 3 * (size_t)ng8, o_M = o_cw + (size_t)G * nw,
+                 total = o_M + nw;
+    std::vector<double> h(total, 0.0);
     for (int i = 0; i < n; ++i) {
-        const long long mi = mods[i];
-        Big Mi{1u};
-        long long rem = 1;
-        for (int j = 0; j < n; ++j)
-            if (j != i) {
-                big_mul_small(Mi, (uint32_t)mods[j]);
-                rem = rem * (mods[j] % mi) % mi;
-            }
-        const long long yi = inv_mod(rem, mi);
-        big_mul_small(Mi, (uint32_t)yi);
-        std::copy(Mi.begin(), Mi.begin() + std::min((int)Mi.size(), nwc), cw.begin() + (size_t)i * nwc);
-        md[i] = (double)mi;
-        inv[i] = 1.0 / (double)mi;
-        frac[i] = (double)yi / (double)mi;
+        h[i] = (double)mods[i];
+        h[o_inv + i] = 1.0 / (double)mods[i];
     }
     for (int i = 0; i < n8; ++i) {
         long long p = 1 % mods[i];
-        for (int h = 0; h < RES_P8; ++h) {
-            p8[(size_t)i * RES_P8 + h] = (uint32_t)p;
-            p = (p << 16) % mods[i];
+        for (int c = 0; c < RES_NC; ++c) {
+            h[o_p37 + (size_t)i * RES_NC + c] = balanced(p, mods[i]);
+            p = (p << 37) % mods[i];   // p < 2^8: no overflow
         }
     }
     for (int j = 0; j < n16; ++j) {
         const long long mj = mods[n8 + j];
         long long p = 1;
-        for (int b = 0; b < RES_PB; ++b) {
-            pb[(size_t)j * RES_PB + b] = (int)(2 * p >= mj ? p - mj : p);
-            p = (p << 8) % mj;
+        for (int c = 0; c < RES_NC; ++c) {
+            const long long b = (long long)balanced(p, mj);   // |b| < 2^15
+            const long long lo = ((b + 128) & 255) - 128;
+            h[o_p37h + (size_t)j * RES_NC + c] = (double)((b - lo) / 256);
+            h[o_p37l + (size_t)j * RES_NC + c] = (double)lo;
+            p = (long long)(((unsigned __int128)p << 37) % (uint64_t)mj);
         }
     }
-    const size_t bd = sizeof(double) * 3 * n, bc = 4 * cw.size(), bm = 4 * Mw.size(), b8 = 4 * p8.size(),
-                 bb = 4 * pb.size();
-    char *d = nullptr;
-    if (hipMalloc(&d, bd + bc + bm + b8 + bb) != hipSuccess) return nullptr;
-    std::vector<char> h(bd + bc + bm + b8 + bb);
-    std::copy((char *)md.data(), (char *)md.data() + 8 * n, h.data());
-    std::copy((char *)inv.data(), (char *)inv.data() + 8 * n, h.data() + 8 * n);
-    std::copy((char *)frac.data(), (char *)frac.data() + 8 * n, h.data() + 16 * n);
-    std::copy((char *)cw.data(), (char *)cw.data() + bc, h.data() + bd);
-    std::copy((char *)Mw.data(), (char *)Mw.data() + bm, h.data() + bd + bc);
-    std::copy((char *)p8.data(), (char *)p8.data() + b8, h.data() + bd + bc + bm);
-    std::copy((char *)pb.data(), (char *)pb.data() + bb, h.data() + bd + bc + bm + b8);
-    if (hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    Big M{1u};
+    for (int m : mods) big_mul_small(M, (uint32_t)m);
+    if (!big_words24(M, nw, &h[o_M])) return nullptr;
+    for (int g = 0; g < G; ++g) {
+        const long long Pg = P[g];
+        Big Mg{1u};
+        long long rem = 1 % Pg;
+        for (int h2 = 0; h2 < G; ++h2)
+            if (h2 != g) {
+                big_mul_small(Mg, (uint32_t)P[h2]);
+                rem = (long long)((unsigned __int128)rem * (uint64_t)(P[h2] % Pg) % (uint64_t)Pg);
+            }
+        const long long yg = inv_mod(rem, Pg);
+        big_mul_small(Mg, (uint32_t)yg);
+        if (!big_words24(Mg, nw, &h[o_cw + (size_t)g * nw])) return nullptr;
+        h[o_gP + g] = (double)Pg;
+        h[o_ginv + g] = 1.0 / (double)Pg;
+        h[o_gfrac + g] = (double)yg / (double)Pg;
+        if (g < ng8)
+            for (size_t x = 0; x < members[g].size(); ++x) {
+                const long long mx = mods[members[g][x]], Px = Pg / mx;
+                h[o_gE + 3 * (size_t)g + x] = (double)(Px * inv_mod(Px % mx, mx));
+            }
+    }
+    double *d = nullptr;
+    if (hipMalloc(&d, total * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), total * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return nullptr;
     }
@@ -488,14 +668,19 @@ static ResTabHost *res_table(int n8, int n16, int nwc)
     r->dev = d;
     r->t.n8 = n8;
     r->t.n16 = n16;
-    r->t.nwc = nwc;
-    r->t.md = (const double *)d;
-    r->t.inv = (const double *)(d + 8 * n);
-    r->t.frac = (const double *)(d + 16 * n);
-    r->t.cw = (const uint32_t *)(d + bd);
-    r->t.Mw = (const uint32_t *)(d + bd + bc);
-    r->t.p8 = (const uint32_t *)(d + bd + bc + bm);
-    r->t.pb = (const int *)(d + bd + bc + bm + b8);
+    r->t.ng8 = ng8;
+    r->t.nw = nw;
+    r->t.md = d;
+    r->t.inv = d + o_inv;
+    r->t.pw37 = d + o_p37;
+    r->t.pw37h = d + o_p37h;
+    r->t.pw37l = d + o_p37l;
+    r->t.gP = d + o_gP;
+    r->t.ginv = d + o_ginv;
+    r->t.gfrac = d + o_gfrac;
+    r->t.gE = d + o_gE;
+    r->t.cw = d + o_cw;
+    r->t.Mw = d + o_M;
     g_res[key] = r;
     return r;
 }
@@ -507,18 +692,18 @@ static int nwu_bucket(int W)
 {
     constexpr int NW = FixedWords<L>::NW;
     const int need = (W + 31) / 32;
-    int b = (int)round_up(need, 4);
     const int lo = (L == 2) ? 4 : (L == 4) ? 8 : (L == 8) ? 12 : 24;
-    return std::min(NW, std::max(lo, b));
+    return std::min(NW, std::max(lo, (int)round_up(need, 4)));
 }
 
-// CRT constant words per limb count: a multiple of 4 in [lo, hi]
-static int nwc_bucket(int L, int words)
+// 24-bit CRT words per limb count: M < 2^mbits and |Z| <= M/4, so the
+// words of M hold Z in two's complement with the carry out as its sign
+static int nw_bucket(int L, int mbits)
 {
-    const int lo = (L == 2) ? 8 : (L == 4) ? 12 : (L == 8) ? 24 : 40;
+    const int words = (mbits + 23) / 24;
+    const int lo = (L == 2) ? 12 : (L == 4) ? 16 : (L == 8) ? 32 : 56;
     const int step = (L <= 4) ? 4 : 8;
-    const int b = (int)round_up(std::max(words, lo), step);
-    return b;
+    return (int)round_up(std::max(words, lo), step);
 }
 
 template <int L, int NWU>
@@ -526,10 +711,12 @@ static void launch_residues(int m, int n, int k, int mp, int np, int kp, const d
                             const double *B, long long ldb, const int *E, const int *F, int W, const ResTab &t,
                             int8_t *A8, int8_t *A16, int8_t *B8, int8_t *B16, hipStream_t s)
 {
-    hipLaunchKernelGGL((residue_a_kernel<L, NWU>), dim3(grid_for((long long)mp * kp, 256)), dim3(256), 0, s, m, k,
-                       mp, kp, A, lda, E, W, t, A8, A16);
-    hipLaunchKernelGGL((residue_b_kernel<L, NWU>), dim3(grid_for((long long)np * kp, 256)), dim3(256), 0, s, k, n,
-                       kp, np, B, ldb, F, W, t, B8, B16);
+    constexpr int EPT = ResEPT<L>::value;
+    constexpr int TC = 256 / (RES_TK / EPT);
+    hipLaunchKernelGGL((residue_a_kernel<L, NWU>), dim3(grid_for((long long)mp * (kp / EPT), 256)), dim3(256), 0, s,
+                       m, k, mp, kp, A, lda, E, W, t, A8, A16);
+    hipLaunchKernelGGL((residue_b_kernel<L, NWU>), dim3((unsigned)((np / TC) * (kp / RES_TK))), dim3(256), 0, s, k,
+                       n, kp, np, B, ldb, F, W, t, B8, B16);
 }
 
 template <int L>
@@ -560,55 +747,57 @@ static int residues_dispatch(int nwu, int m, int n, int k, int mp, int np, int k
     return MDG_ERROR_RANGE;
 }
 
-template <int L, int NWC>
-static void launch_reconstruct(int m, int n, int W, const ResTab &t, int nck1, int nck2, const int *D8,
+template <int L, int NW>
+static void launch_reconstruct(int m, int n, int W, const ResTab &t, int nck1, int nck2, long long ps, int reduce,
+                               const int *D8,
                                const int *Dhh, const int *Dll, const int *Dx, const int *E, const int *F, double *C,
                                long long ldc, hipStream_t s)
 {
-    hipLaunchKernelGGL((residue_reconstruct_kernel<L, NWC>), dim3(grid_for((long long)m * n, 256)), dim3(256), 0, s,
-                       m, n, W, t, nck1, nck2, D8, Dhh, Dll, Dx, E, F, C, ldc);
+    hipLaunchKernelGGL((residue_reconstruct_kernel<L, NW>), dim3(grid_for((long long)m * n, 256)), dim3(256), 0, s,
+                       m, n, W, t, nck1, nck2, ps, reduce, D8, Dhh, Dll, Dx, E, F, C, ldc);
 }
 
 template <int L>
-static int reconstruct_dispatch(int nwc, int m, int n, int W, const ResTab &t, int nck1, int nck2, const int *D8,
+static int reconstruct_dispatch(int nw, int m, int n, int W, const ResTab &t, int nck1, int nck2, long long ps,
+                                int reduce, const int *D8,
                                 const int *Dhh, const int *Dll, const int *Dx, const int *E, const int *F,
                                 double *C, long long ldc, hipStream_t s)
 {
-#define MDG_REC_CASE(NWC)                                                                              \
-    case NWC:                                                                                          \
-        launch_reconstruct<L, NWC>(m, n, W, t, nck1, nck2, D8, Dhh, Dll, Dx, E, F, C, ldc, s);         \
+#define MDG_REC_CASE(NW)                                                                               \
+    case NW:                                                                                           \
+        launch_reconstruct<L, NW>(m, n, W, t, nck1, nck2, ps, reduce, D8, Dhh, Dll, Dx, E, F, C, ldc, s); \
         return MDG_SUCCESS;
     if constexpr (L == 2) {
-        switch (nwc) { MDG_REC_CASE(8) MDG_REC_CASE(12) MDG_REC_CASE(16) MDG_REC_CASE(20) }
+        switch (nw) { MDG_REC_CASE(12) MDG_REC_CASE(16) MDG_REC_CASE(20) MDG_REC_CASE(24) }
     } else if constexpr (L == 4) {
-        switch (nwc) { MDG_REC_CASE(12) MDG_REC_CASE(16) MDG_REC_CASE(20) MDG_REC_CASE(24) MDG_REC_CASE(28) }
+        switch (nw) {
+            MDG_REC_CASE(16) MDG_REC_CASE(20) MDG_REC_CASE(24) MDG_REC_CASE(28) MDG_REC_CASE(32) MDG_REC_CASE(36)
+        }
     } else if constexpr (L == 8) {
-        switch (nwc) { MDG_REC_CASE(24) MDG_REC_CASE(32) MDG_REC_CASE(40) MDG_REC_CASE(48) }
+        switch (nw) { MDG_REC_CASE(32) MDG_REC_CASE(40) MDG_REC_CASE(48) MDG_REC_CASE(56) }
     } else {
-        switch (nwc) {
-            MDG_REC_CASE(40) MDG_REC_CASE(48) MDG_REC_CASE(56) MDG_REC_CASE(64) MDG_REC_CASE(72) MDG_REC_CASE(80)
+        switch (nw) {
+            MDG_REC_CASE(56) MDG_REC_CASE(64) MDG_REC_CASE(72) MDG_REC_CASE(80) MDG_REC_CASE(88) MDG_REC_CASE(96)
+            MDG_REC_CASE(104)
         }
     }
 #undef MDG_REC_CASE
-    set_error("no reconstruction kernel for %d CRT words at L=%d", nwc, L);
+    set_error("no reconstruction kernel for %d CRT words at L=%d", nw, L);
     return MDG_ERROR_RANGE;
 }
 
-// one strided-batched INT8 GEMM per K-chunk: D[b][q] (n x m col-major view of
-// the row-major m x n plane) = Bp[b]^T-op * Ap[b] over depth [k0, k0 + kc)
 static int res_batched(rocblas_handle h, int batch, int m, int n, int depth, const int8_t *Bp, long long ldb,
-                       long long sb, const int8_t *Ap, long long lda, long long sa, int *D, int nck)
+                       long long sb, const int8_t *Ap, long long lda, long long sa, int *D, long long ps, int nck)
 {
     const int32_t alpha = 1, beta = 0;
-    const long long mn = (long long)m * n;
     for (int q = 0; q * (long long)RES_KC < depth; ++q) {
         const int k0 = q * RES_KC;
         const int kc = std::min(RES_KC, depth - k0);
-        int *Dq = D + (long long)q * mn;
+        int *Dq = D + (long long)q * ps;
         rocblas_status st = rocblas_gemm_strided_batched_ex(
             h, rocblas_operation_transpose, rocblas_operation_none, n, m, kc, &alpha, Bp + k0, rocblas_datatype_i8_r,
-            ldb, sb, Ap + k0, rocblas_datatype_i8_r, lda, sa, &beta, Dq, rocblas_datatype_i32_r, n, mn * nck, Dq,
-            rocblas_datatype_i32_r, n, mn * nck, batch, rocblas_datatype_i32_r, rocblas_gemm_algo_standard, 0, 0);
+            ldb, sb, Ap + k0, rocblas_datatype_i8_r, lda, sa, &beta, Dq, rocblas_datatype_i32_r, n, ps * nck, Dq,
+            rocblas_datatype_i32_r, n, ps * nck, batch, rocblas_datatype_i32_r, rocblas_gemm_algo_standard, 0, 0);
         if (st != rocblas_status_success) {
             set_error("rocblas_gemm_strided_batched_ex (int8) failed with status %d", (int)st);
             return MDG_ERROR_HIP;
@@ -642,24 +831,24 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
     int n8 = 0, n16 = 0, mbits = 0;
     MDG_REQUIRE(choose_moduli(need, n8, n16, &mbits), MDG_ERROR_RANGE, "no modulus set reaches %d bits", need);
     const int nwu = nwu_bucket<L>(W);
-    const int nwc = nwc_bucket(L, (mbits + 31) / 32 + 1);   // M's words and a sign word
-    ResTabHost *th = res_table(n8, n16, nwc);
+    const int nw = nw_bucket(L, mbits);
+    ResTabHost *th = res_table(n8, n16, nw);
     MDG_REQUIRE(th, MDG_ERROR_HIP, "could not build the modulus tables (%d + %d moduli)", n8, n16);
     const ResTab &t = th->t;
 
     const int kp = (int)round_up(k, 128), mp = (int)round_up(m, 128), np = (int)round_up(n, 128);
     const int nck1 = (kp + RES_KC - 1) / RES_KC, nck2 = (2 * kp + RES_KC - 1) / RES_KC;
-    const long long mn = (long long)m * n;
+    const long long ps = plane_stride((long long)m * n, 4);
     const size_t a8 = (size_t)n8 * mp * kp, a16 = (size_t)n16 * mp * 2 * kp;
     const size_t b8 = (size_t)n8 * np * kp, b16 = (size_t)n16 * np * 3 * kp;
     const size_t nplanes = (size_t)n8 * nck1 + (size_t)n16 * (2 * nck1 + nck2);
     int8_t *Ares = (int8_t *)scratch(s, a8 + a16 + 256, 2);
     int8_t *Bres = (int8_t *)scratch(s, b8 + b16 + 256, 3);
-    int *D = (int *)scratch(s, nplanes * mn * 4, 4);
+    int *D = (int *)scratch(s, nplanes * ps * 4, 4);
     MDG_REQUIRE(Ares && Bres && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     int8_t *A8 = Ares, *A16 = Ares + a8, *B8 = Bres, *B16 = Bres + b8;
-    int *D8 = D, *Dhh = D + (size_t)n8 * nck1 * mn, *Dll = Dhh + (size_t)n16 * nck1 * mn,
-        *Dx = Dll + (size_t)n16 * nck1 * mn;
+    int *D8 = D, *Dhh = D + (size_t)n8 * nck1 * ps, *Dll = Dhh + (size_t)n16 * nck1 * ps,
+        *Dx = Dll + (size_t)n16 * nck1 * ps;
 
     rc = residues_dispatch<L>(nwu, m, n, k, mp, np, kp, A, lda, B, ldb, E, F, W, t, A8, A16, B8, B16, s);
     if (rc) return rc;
@@ -675,21 +864,21 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
         }
         const long long sa8 = (long long)mp * kp, sb8 = (long long)np * kp;
         const long long sa16 = (long long)mp * 2 * kp, sb16 = (long long)np * 3 * kp;
-        if (n8 && (rc = res_batched(bh->h, n8, m, n, kp, B8, kp, sb8, A8, kp, sa8, D8, nck1))) return rc;
+        if (n8 && (rc = res_batched(bh->h, n8, m, n, kp, B8, kp, sb8, A8, kp, sa8, D8, ps, nck1))) return rc;
         if (n16) {
             // hi*hi, lo*lo, and (hi, lo) . (lo, hi) of depth 2kp
-            if ((rc = res_batched(bh->h, n16, m, n, kp, B16, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dhh, nck1)))
+            if ((rc = res_batched(bh->h, n16, m, n, kp, B16, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dhh, ps, nck1)))
                 return rc;
-            if ((rc = res_batched(bh->h, n16, m, n, kp, B16 + kp, 3ll * kp, sb16, A16 + kp, 2ll * kp, sa16, Dll,
+            if ((rc = res_batched(bh->h, n16, m, n, kp, B16 + kp, 3ll * kp, sb16, A16 + kp, 2ll * kp, sa16, Dll, ps,
                                   nck1)))
                 return rc;
-            if ((rc = res_batched(bh->h, n16, m, n, 2 * kp, B16 + kp, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dx,
+            if ((rc = res_batched(bh->h, n16, m, n, 2 * kp, B16 + kp, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dx, ps,
                                   nck2)))
                 return rc;
         }
     }
     tm.mark(3);
-    rc = reconstruct_dispatch<L>(nwc, m, n, W, t, nck1, nck2, D8, Dhh, Dll, Dx, E, F, C, ldc, s);
+    rc = reconstruct_dispatch<L>(nw, m, n, W, t, nck1, nck2, ps, (nck1 > 1 || kp > 8192) ? 1 : 0, D8, Dhh, Dll, Dx, E, F, C, ldc, s);
     if (rc) return rc;
     MDG_LAUNCH_CHECK("residue_reconstruct_kernel");
     tm.mark(4);

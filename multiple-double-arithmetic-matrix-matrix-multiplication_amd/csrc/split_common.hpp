@@ -30,4 +30,14 @@ inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q
 
 inline unsigned grid_for(long long nel, int bs) { return (unsigned)((nel + bs - 1) / bs); }
 
+// Element stride between consecutive m x n result planes.  A power-of-two
+// plane size puts the same offset of every plane on the same HBM channels,
+// so a kernel that streams many planes at once (the reconstructions) runs
+// at a fraction of the bandwidth; an odd multiple of 256 bytes of padding
+// spreads the planes over the channels.
+inline long long plane_stride(long long mn, int elem_bytes)
+{
+    return (round_up(mn * elem_bytes, 256) + 37 * 256) / elem_bytes;
+}
+
 }  // namespace mdg
