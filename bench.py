@@ -40,9 +40,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--paths", default="int8,int8_mfma,f64,bf16,vector",
+    ap.add_argument("--paths", default="int8,int8_residue,int8_mfma,f64,bf16,vector",
                     help="int8 = INT8 slices with the autotuned diagonal-GEMM engine, int8_mfma = "
-                         "forced to this library's INT8 MFMA kernel")
+                         "forced to this library's INT8 MFMA kernel, int8_residue = INT8 residues "
+                         "modulo coprime moduli (CRT), f64 = FP64-MFMA split, bf16 = BF16 slices, "
+                         "vector = one thread per output entry")
     return ap.parse_args()
 
 
@@ -87,7 +89,7 @@ def main():
     outputs = {}
     mdg.set_profiling(True)
     for path in paths:
-        steps = a.steps if path == "int8" else (1 if path == "vector" else max(1, a.steps // 4))
+        steps = a.steps if path in ("int8", "int8_residue") else (1 if path == "vector" else max(1, a.steps // 4))
         warm = a.warmup if path != "vector" else 0
         if path == "vector":
             # load the kernel on a small problem first
@@ -113,7 +115,7 @@ def main():
                          "slices": info["slices"], "slice_bits": info["slice_bits"],
                          "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
                          "spread": [info["spread_a"], info["spread_b"]], "steps": steps,
-                         "engine": {1: "rocblas", 2: "mfma"}.get(info["backend"], "vector")}
+                         "engine": {1: "rocblas", 2: "mfma", 3: "hipblaslt"}.get(info["backend"], "vector")}
         log(f"{path}: {ms:.2f} ms/step, {eff_flops / ms / 1e6:.1f} GFLOP/s effective, "
             f"stages {np.round(stage, 3).tolist()} ms")
     mdg.set_profiling(False)
@@ -137,7 +139,8 @@ def main():
     if head_path == "int8":
         work = P * 2.0 * m * n * k
         peak, unit = INT8_PEAK_TOPS, "TFLOP/s"
-        kernel = "rocblas_int8_gemm" if head["engine"] == "rocblas" else "diag_i8_kernel"
+        # the library engines (rocBLAS, hipBLASLt) both run Tensile INT8 kernels
+        kernel = "diag_i8_kernel" if head["engine"] == "mfma" else "int8_plane_gemm"
     elif head_path == "bf16":
         work = P * 2.0 * m * n * k
         peak, unit = BF16_PEAK_TFLOPS, "TFLOP/s"
@@ -186,6 +189,12 @@ def main():
                          f"scaled by effective 2*rows*n*k flops"}
         log(f"cpu baseline: {cpu_gflops:.3f} GFLOP/s on {nthreads} threads ({rows} rows)")
 
+    def ratio(a_, b_):
+        return results[a_]["gflops_eff"] / results[b_]["gflops_eff"] if a_ in results and b_ in results else None
+
+    ratios = {"int8_over_f64": ratio("int8", "f64"), "f64_over_vector": ratio("f64", "vector"),
+              "int8_residue_over_int8": ratio("int8_residue", "int8"),
+              "int8_residue_over_f64": ratio("int8_residue", "f64")}
     line = {
         "metric": "effective qd-GEMM GFLOP/s at n=4096: INT8-slice vs FP64-MFMA split vs vector, same err",
         "value": head["gflops_eff"],
@@ -203,6 +212,7 @@ def main():
                                f"e in [-8,8], seeds A={seeds[0]} B={seeds[1]}",
                    "L": L, "m": m, "n": n, "k": k, "path": head_path},
         "paths": results,
+        "ratios": ratios,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -71,7 +71,7 @@ typedef struct mdg_gemm_info {
        profiling is on (mdg_set_profiling), else 0:
        [0] exponent scan, [1] slicing, [2] diagonal GEMMs, [3] reconstruction */
     float stage_ms[4];
-    int backend;           /* diagonal GEMM engine: 1 rocBLAS (INT8 only),
+    int backend;           /* GEMM engine: 1 rocBLAS, 3 hipBLASLt (INT8 only),
                               2 this library's MFMA kernels */
 } mdg_gemm_info;
 
@@ -100,9 +100,12 @@ int mdg_release_workspace(void);
    when on, mdg_gemm waits for its own last event before returning. */
 int mdg_set_profiling(int on);
 
-/* Engine of the INT8 diagonal GEMMs: 0 = default (rocBLAS), 1 = rocBLAS
-   gemm_ex (INT8 x INT8 -> INT32), 2 = this library's MFMA kernel.  Both are
-   exact; they differ only in speed.  Process-wide. */
+/* Engine of the INT8 GEMMs (slice planes and residue products):
+   0 = autotuned per shape (the first call of a shape times the engines and
+   keeps the fastest), 1 = rocBLAS, 2 = this library's MFMA kernel (slice
+   planes; the residue products then autotune between 1 and 3),
+   3 = hipBLASLt (its fastest proposed algorithm).  All are exact and give
+   identical results; they differ only in speed.  Process-wide. */
 int mdg_set_int8_backend(int backend);
 
 /* ----------------------------------------------------------------------

@@ -786,22 +786,18 @@ static int reconstruct_dispatch(int nw, int m, int n, int W, const ResTab &t, in
     return MDG_ERROR_RANGE;
 }
 
-static int res_batched(rocblas_handle h, int batch, int m, int n, int depth, const int8_t *Bp, long long ldb,
-                       long long sb, const int8_t *Ap, long long lda, long long sa, int *D, long long ps, int nck)
+// one strided-batched INT8 GEMM per K-chunk of the depth; chunk q of batch
+// entry b lands in plane b * nck + q (plane stride ps)
+static int res_batched(int batch, int m, int n, int depth, const int8_t *Bp, long long ldb, long long sb,
+                       const int8_t *Ap, long long lda, long long sa, int *D, long long ps, int nck, hipStream_t s,
+                       int *engine)
 {
-    const int32_t alpha = 1, beta = 0;
     for (int q = 0; q * (long long)RES_KC < depth; ++q) {
         const int k0 = q * RES_KC;
         const int kc = std::min(RES_KC, depth - k0);
-        int *Dq = D + (long long)q * ps;
-        rocblas_status st = rocblas_gemm_strided_batched_ex(
-            h, rocblas_operation_transpose, rocblas_operation_none, n, m, kc, &alpha, Bp + k0, rocblas_datatype_i8_r,
-            ldb, sb, Ap + k0, rocblas_datatype_i8_r, lda, sa, &beta, Dq, rocblas_datatype_i32_r, n, ps * nck, Dq,
-            rocblas_datatype_i32_r, n, ps * nck, batch, rocblas_datatype_i32_r, rocblas_gemm_algo_standard, 0, 0);
-        if (st != rocblas_status_success) {
-            set_error("rocblas_gemm_strided_batched_ex (int8) failed with status %d", (int)st);
-            return MDG_ERROR_HIP;
-        }
+        const int rc = i8_gemm_tn(batch, m, n, kc, Bp + k0, ldb, sb, Ap + k0, lda, sa, D + (long long)q * ps, n,
+                                  ps * nck, s, engine);
+        if (rc) return rc;
     }
     return MDG_SUCCESS;
 }
@@ -854,31 +850,27 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
     if (rc) return rc;
     MDG_LAUNCH_CHECK("residue kernels");
     tm.mark(2);
+    int engine = 1, wide_engine = 1;
     {
-        BlasHandle *bh = blas_handle();
-        MDG_REQUIRE(bh, MDG_ERROR_HIP, "rocblas_create_handle failed");
-        std::lock_guard<std::mutex> lock(bh->mu);
-        if (rocblas_set_stream(bh->h, s) != rocblas_status_success) {
-            set_error("rocblas_set_stream failed");
-            return MDG_ERROR_HIP;
-        }
         const long long sa8 = (long long)mp * kp, sb8 = (long long)np * kp;
         const long long sa16 = (long long)mp * 2 * kp, sb16 = (long long)np * 3 * kp;
-        if (n8 && (rc = res_batched(bh->h, n8, m, n, kp, B8, kp, sb8, A8, kp, sa8, D8, ps, nck1))) return rc;
+        if (n8 && (rc = res_batched(n8, m, n, kp, B8, kp, sb8, A8, kp, sa8, D8, ps, nck1, s, &engine))) return rc;
         if (n16) {
             // hi*hi, lo*lo, and (hi, lo) . (lo, hi) of depth 2kp
-            if ((rc = res_batched(bh->h, n16, m, n, kp, B16, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dhh, ps, nck1)))
+            if ((rc = res_batched(n16, m, n, kp, B16, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dhh, ps, nck1, s,
+                                  &wide_engine)))
                 return rc;
-            if ((rc = res_batched(bh->h, n16, m, n, kp, B16 + kp, 3ll * kp, sb16, A16 + kp, 2ll * kp, sa16, Dll, ps,
-                                  nck1)))
+            if ((rc = res_batched(n16, m, n, kp, B16 + kp, 3ll * kp, sb16, A16 + kp, 2ll * kp, sa16, Dll, ps, nck1,
+                                  s, &wide_engine)))
                 return rc;
-            if ((rc = res_batched(bh->h, n16, m, n, 2 * kp, B16 + kp, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dx, ps,
-                                  nck2)))
+            if ((rc = res_batched(n16, m, n, 2 * kp, B16 + kp, 3ll * kp, sb16, A16, 2ll * kp, sa16, Dx, ps, nck2, s,
+                                  &wide_engine)))
                 return rc;
         }
     }
     tm.mark(3);
-    rc = reconstruct_dispatch<L>(nw, m, n, W, t, nck1, nck2, ps, (nck1 > 1 || kp > 8192) ? 1 : 0, D8, Dhh, Dll, Dx, E, F, C, ldc, s);
+    rc = reconstruct_dispatch<L>(nw, m, n, W, t, nck1, nck2, ps, (nck1 > 1 || kp > 8192) ? 1 : 0, D8, Dhh, Dll, Dx,
+                                 E, F, C, ldc, s);
     if (rc) return rc;
     MDG_LAUNCH_CHECK("residue_reconstruct_kernel");
     tm.mark(4);
@@ -894,7 +886,7 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
         info->spread_a = spread_a;
         info->spread_b = spread_b;
         info->flushed = (nck1 > 1 || nck2 > 1) ? 1 : 0;
-        info->backend = 1;
+        info->backend = engine;   // engine of the narrow-modulus batch, the largest
         for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;

@@ -18,6 +18,14 @@ struct BlasHandle {
 };
 BlasHandle *blas_handle();
 
+// INT8 x INT8 -> INT32 strided-batched GEMM on rocBLAS or hipBLASLt, the
+// engine (and hipBLASLt algorithm) autotuned per shape unless forced by
+// mdg_set_int8_backend.  Column-major view: D_b (n x m, ldd) = Bt_b^T * A_b
+// with Bt_b[c * ldb + kk], A_b[r * lda + kk], kk < depth (row-major: the
+// m x n product of A_b and Bt_b^T).  *engine_used: 1 rocBLAS, 3 hipBLASLt.
+int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long ldb, long long sb, const int8_t *A,
+               long long lda, long long sa, int *D, long long ldd, long long sd, hipStream_t s, int *engine_used);
+
 // Scale exponents of A's rows (*E, m ints) and B's columns (*F, n ints) in
 // scratch slots 0 and 1: every |entry| < 2^E, all-zero rows/columns get
 // ZERO_SCALE.  With fixed_spread < 0 the exponent spreads are measured and

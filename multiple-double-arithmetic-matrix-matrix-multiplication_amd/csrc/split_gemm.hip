@@ -716,7 +716,7 @@ BlasHandle *blas_handle()
 
 static int int8_backend_choice()
 {
-    // 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel
+    // 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel, 3 hipBLASLt
     return int8_backend();
 }
 
@@ -749,17 +749,11 @@ static void remember_engine(const TuneKey &k, int engine)
     g_tune[bucket(k)] = engine;
 }
 
-static int i8_planes_rocblas(const int8_t *Aall, const int8_t *Ball, int S, int kp, int kmax, int m, int n, int *D,
-                             hipStream_t s)
+// every K-chunk plane as one INT8 GEMM on the library engines (rocBLAS or
+// hipBLASLt, autotuned per plane shape); *engine: the engine of the last plane
+static int i8_planes_library(const int8_t *Aall, const int8_t *Ball, int S, int kp, int kmax, int m, int n, int *D,
+                             hipStream_t s, int *engine)
 {
-    BlasHandle *b = blas_handle();
-    MDG_REQUIRE(b, MDG_ERROR_HIP, "rocblas_create_handle failed");
-    std::lock_guard<std::mutex> lock(b->mu);
-    if (rocblas_set_stream(b->h, s) != rocblas_status_success) {
-        set_error("rocblas_set_stream failed");
-        return MDG_ERROR_HIP;
-    }
-    const int32_t alpha = 1, beta = 0;
     const long long ld = (long long)S * kp;
     const long long mn = (long long)m * n;
     int p = 0;
@@ -767,16 +761,9 @@ static int i8_planes_rocblas(const int8_t *Aall, const int8_t *Ball, int S, int 
         const int K = (g + 1) * kp;
         for (int k0 = 0; k0 < K; k0 += kmax, ++p) {
             const int kk = std::min(kmax, K - k0);
-            int *Dp = D + (long long)p * mn;
-            rocblas_status st = rocblas_gemm_ex(
-                b->h, rocblas_operation_transpose, rocblas_operation_none, n, m, kk, &alpha,
-                Ball + (long long)(S - 1 - g) * kp + k0, rocblas_datatype_i8_r, ld, Aall + k0, rocblas_datatype_i8_r,
-                ld, &beta, Dp, rocblas_datatype_i32_r, n, Dp, rocblas_datatype_i32_r, n, rocblas_datatype_i32_r,
-                rocblas_gemm_algo_standard, 0, 0);
-            if (st != rocblas_status_success) {
-                set_error("rocblas_gemm_ex (int8) failed with status %d", (int)st);
-                return MDG_ERROR_HIP;
-            }
+            const int rc = i8_gemm_tn(1, m, n, kk, Ball + (long long)(S - 1 - g) * kp + k0, ld, 0, Aall + k0, ld, 0,
+                                      D + (long long)p * mn, n, mn, s, engine);
+            if (rc) return rc;
         }
     }
     return MDG_SUCCESS;
@@ -793,7 +780,7 @@ struct Plan {
     bool wide;        // INT8: widened accumulation
     int flush_blocks; // FP64 / INT8
     int spread_a, spread_b;
-    int backend;      // INT8: 1 rocBLAS, 2 hand-written MFMA kernel
+    int backend;      // INT8: 1 library engine (rocBLAS / hipBLASLt), 2 hand-written MFMA kernel
 };
 
 // Choose slice width and count for a precision target and the measured
@@ -928,6 +915,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     const unsigned zdim = (unsigned)S;
     MDG_REQUIRE(zdim <= 65535u, MDG_ERROR_RANGE, "too many slices (%d)", S);
 
+    int lib_engine_used = 1;
     if (p.method == MDG_METHOD_INT8_OZAKI) {
         // INT32 planes exact while chunk depth K * 2^14 < 2^31
         const int kmax = (kp <= 131071) ? (131071 / kp) * kp : 130944;
@@ -945,8 +933,9 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
                            s, k, n, kp, np, B, ldb, F, S, w, Ball);
         tm.mark(2);
+        int lib_engine = 1;   // rocBLAS or hipBLASLt, as the library planes ran
         auto run_engine = [&](int engine) -> int {
-            if (engine == 1) return i8_planes_rocblas(Aall, Ball, S, kp, kmax, m, n, D, s);
+            if (engine == 1) return i8_planes_library(Aall, Ball, S, kp, kmax, m, n, D, s, &lib_engine);
             MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
             const int tiles_m8 = mp8 / 256, tiles_n = np / 128;
             dim3 grid(tiles_m8 * tiles_n, 1, (unsigned)nplanes);
@@ -957,8 +946,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         };
         const int be = int8_backend_choice();
         bool planes_done = false;
-        if (be == 1 || be == 2) {
-            p.backend = be;
+        if (be == 1 || be == 2 || be == 3) {
+            p.backend = (be == 2) ? 2 : 1;   // forced library engine: i8_gemm_tn honours it
         } else {
             // autotuned: the first call of a shape bucket times both engines on
             // the same slices (they produce identical planes) and keeps the faster
@@ -993,6 +982,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             rc = run_engine(p.backend);
             if (rc) return rc;
         }
+        lib_engine_used = lib_engine;
         tm.mark(3);
         hipLaunchKernelGGL((reconstruct_i8_kernel<L>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, kp,
                            kmax, w, (const int *)D, E, F, C, ldc);
@@ -1050,7 +1040,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         info->spread_a = p.spread_a;
         info->spread_b = p.spread_b;
         info->flushed = p.wide ? 1 : 0;
-        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? p.backend : 2;
+        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? (p.backend == 1 ? lib_engine_used : 2) : 2;
         for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;

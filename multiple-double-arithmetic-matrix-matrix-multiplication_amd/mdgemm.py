@@ -116,7 +116,8 @@ def set_profiling(on):
 
 
 def set_int8_backend(backend):
-    """0 default (rocBLAS), 1 rocBLAS, 2 hand-written MFMA kernel."""
+    """INT8 GEMM engine: 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel
+    (slice planes), 3 hipBLASLt."""
     _check(lib().mdg_set_int8_backend(int(backend)))
 
 

@@ -9,14 +9,14 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "bf16", "int8_residue"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "int8_hipblaslt", "bf16", "int8_residue"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
-    """int8 runs the autotuned engine, int8_rocblas rocBLAS gemm_ex and
-    int8_mfma this library's INT8 MFMA kernel; all give the same exact
-    diagonal sums."""
-    engine = {"int8_rocblas": 1, "int8_mfma": 2}.get(method)
+    """int8 runs the autotuned engine, int8_rocblas rocBLAS, int8_mfma this
+    library's INT8 MFMA kernel and int8_hipblaslt hipBLASLt; all give the
+    same exact diagonal sums."""
+    engine = {"int8_rocblas": 1, "int8_mfma": 2, "int8_hipblaslt": 3}.get(method)
     if engine:
         mdg.set_int8_backend(engine)
         try:
@@ -132,13 +132,15 @@ def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
     B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 18, -20, 20))
     C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8_rocblas")
     C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
-    assert i1["backend"] == 1 and i2["backend"] == 2
+    C4, i4 = gemm(mdg, L, m, n, k, A, B, "int8_hipblaslt")
+    assert i1["backend"] == 1 and i2["backend"] == 2 and i4["backend"] == 3
     assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
+    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C4).view(np.uint64))
     # the autotuner times both engines on the first call of a shape bucket,
     # then keeps its choice; every call gives the same bits
     for _ in range(2):
         C3, i3 = gemm(mdg, L, m, n, k, A, B, "int8")
-        assert i3["backend"] in (1, 2)
+        assert i3["backend"] in (1, 2, 3)
         assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
 
 
@@ -153,3 +155,21 @@ def test_residue_chunked_depth(mdg, orc, torch_cuda):
     err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=96)
     assert err <= _bound(L, k), (err, info)
 
+
+
+@pytest.mark.parametrize("L", [2, 4])
+def test_residue_engines_agree_bitwise(mdg, orc, torch_cuda, L):
+    # the residue products are exact integers on every engine
+    m, n, k = 200, 150, 300
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 27, -20, 20))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 28, -20, 20))
+    out = {}
+    for engine in (1, 3, 0):
+        mdg.set_int8_backend(engine)
+        try:
+            C, info = mdg.gemm(L, m, n, k, A, B, method="int8_residue")
+        finally:
+            mdg.set_int8_backend(0)
+        assert info["backend"] == (engine or info["backend"]) and info["backend"] in (1, 3)
+        out[engine] = to_host(C).view(np.uint64)
+    assert np.array_equal(out[1], out[3]) and np.array_equal(out[1], out[0])
