@@ -40,10 +40,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--paths", default="int8,int8_residue,int8_mfma,f64,bf16,vector",
+    ap.add_argument("--paths", default="int8,int8_residue,f64_residue,int8_mfma,f64,bf16,vector",
                     help="int8 = INT8 slices with the autotuned diagonal-GEMM engine, int8_mfma = "
                          "forced to this library's INT8 MFMA kernel, int8_residue = INT8 residues "
-                         "modulo coprime moduli (CRT), f64 = FP64-MFMA split, bf16 = BF16 slices, "
+                         "modulo coprime moduli (CRT), f64_residue = the same on FP64 MFMA with 21-bit "
+                         "primes, f64 = FP64-MFMA split, bf16 = BF16 slices, "
                          "vector = one thread per output entry")
     return ap.parse_args()
 
@@ -194,7 +195,8 @@ def main():
 
     ratios = {"int8_over_f64": ratio("int8", "f64"), "f64_over_vector": ratio("f64", "vector"),
               "int8_residue_over_int8": ratio("int8_residue", "int8"),
-              "int8_residue_over_f64": ratio("int8_residue", "f64")}
+              "int8_residue_over_f64_residue": ratio("int8_residue", "f64_residue"),
+              "f64_residue_over_vector": ratio("f64_residue", "vector")}
     line = {
         "metric": "effective qd-GEMM GFLOP/s at n=4096: INT8-slice vs FP64-MFMA split vs vector, same err",
         "value": head["gflops_eff"],

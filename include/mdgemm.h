@@ -41,9 +41,12 @@ typedef enum mdg_method {
     MDG_METHOD_F64_SPLIT = 1,   /* signed fixed-point slices on FP64 MFMA */
     MDG_METHOD_INT8_OZAKI = 2,  /* 8-bit slices on INT8 MFMA, INT32 accumulate */
     MDG_METHOD_BF16_OZAKI = 3,  /* 8-bit slices on BF16 MFMA, FP32 accumulate */
-    MDG_METHOD_INT8_RESIDUE = 4 /* INT8 residues modulo coprime moduli on INT8
-                                   MFMA, exact CRT reconstruction (no
-                                   counterpart in the reference) */
+    MDG_METHOD_INT8_RESIDUE = 4, /* INT8 residues modulo coprime moduli on INT8
+                                    MFMA, exact CRT reconstruction (no
+                                    counterpart in the reference) */
+    MDG_METHOD_F64_RESIDUE = 5   /* residues modulo primes below 2^21 on FP64
+                                    MFMA, exact CRT reconstruction (no
+                                    counterpart in the reference) */
 } mdg_method;
 
 /* two_prod flavour of the vector path */
@@ -127,11 +130,11 @@ int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda,
  * it replaces the reference's split -> tensor-core GEMM -> squeeze chain,
  * C++/TCtest.cu:53-114, for signed inputs of any exponent range).
  *   method       MDG_METHOD_F64_SPLIT / _INT8_OZAKI / _BF16_OZAKI /
- *                _INT8_RESIDUE, or MDG_METHOD_VECTOR (then identical to
- *                mdg_vector_gemm, FMA).  _INT8_RESIDUE computes the same
- *                fixed-point product from residues modulo pairwise coprime
- *                moduli (Ozaki scheme II): INT8 GEMMs linear, not quadratic,
- *                in the bit count; k <= 2^22.
+ *                _INT8_RESIDUE / _F64_RESIDUE, or MDG_METHOD_VECTOR (then
+ *                identical to mdg_vector_gemm, FMA).  The _RESIDUE methods
+ *                compute the fixed-point product from residues modulo
+ *                pairwise coprime moduli (Ozaki scheme II): GEMMs linear,
+ *                not quadratic, in the bit count; k <= 2^22.
  *   target_bits  fixed-point precision goal; <= 0 means 53*L.  The slice
  *                count is chosen from it and from the exponent spread of
  *                each operand's rows/columns (a small device->host read).

@@ -193,7 +193,7 @@ int mdg_gemm(int L, int m, int n, int k, const double *A, long long lda, const d
 {
     int rc = check_gemm_args(L, m, n, k, A, lda, B, ldb, C, ldc);
     if (rc) return rc;
-    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_INT8_RESIDUE, MDG_ERROR_INVALID_VALUE,
+    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_F64_RESIDUE, MDG_ERROR_INVALID_VALUE,
                 "unknown method %d", method);
     if ((rc = ensure_device())) return rc;
     if (info) {
@@ -235,7 +235,7 @@ int mdg_gemm_batched(int count, const mdg_problem *problems, int method, int tar
             return rc;
         }
     }
-    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_INT8_RESIDUE, MDG_ERROR_INVALID_VALUE,
+    MDG_REQUIRE(method >= MDG_METHOD_VECTOR && method <= MDG_METHOD_F64_RESIDUE, MDG_ERROR_INVALID_VALUE,
                 "unknown method %d", method);
     int rc = ensure_device();
     if (rc) return rc;

@@ -43,6 +43,7 @@
 namespace mdg {
 
 constexpr int RES_NC = 32;       // 37-bit chunks of the widest fixed-point integer (36 words)
+constexpr int RES_NC24 = 48;     // its 24-bit chunks
 constexpr int RES_KC = 130944;   // GEMM depth with exact INT32 sums: RES_KC * 2^14 < 2^31, multiple of 128
 constexpr int RES_TK = 64;       // B residue tile depth (k rows)
 
@@ -64,6 +65,10 @@ struct ResTab {
     const double *gE;           // narrow group g: E_x = (P/m_x)((P/m_x)^-1 mod m_x) of its members, 0 if absent [ng8][3]
     const double *cw;           // c_g = (M/P_g) y_g, [G][nw] 24-bit words, least significant first
     const double *Mw;           // M, [nw] 24-bit words
+    // FP64-MFMA moduli (odd primes below 2^21; then n8 = n16 = 0, G = nf)
+    int nf;
+    const double *pw24;         // balanced 2^(24c) mod m_i                              [nf][RES_NC24]
+    const double *c32;          // balanced 2^32 mod m_i                                 [nf]
 };
 
 // v mod m in [0, m) for an integer-valued double |v| < 2^53: the FP64
@@ -273,6 +278,95 @@ __global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, in
                                 R16 + (long long)c * 3 * kp + kk, 3ll * np * kp, kp, true);
 }
 
+// ----------------------------------------------- FP64-MFMA residue planes --
+// bits [24c, 24c + 24) of the non-negative NW-word integer N, as a double
+template <int NW>
+__device__ __forceinline__ double chunk24(const uint32_t (&N)[NW], int c)
+{
+    const int pos = 24 * c, j = pos >> 5, off = pos & 31;
+    const uint32_t w0 = N[j], w1 = (j + 1 < NW) ? N[j + 1] : 0u;
+    return (double)(__builtin_amdgcn_alignbit(w1, w0, off) & 0xFFFFFFu);
+}
+
+// Balanced residues of one entry modulo the FP64 moduli (|r| < 2^20), one
+// INT32 plane per modulus at o + i * plane: 24-bit chunks of |N| against
+// balanced 2^(24c) mod m (< 2^20): at most 48 terms of 2^44, below 2^50.
+template <int L, int NWU>
+__device__ __forceinline__ void f64_entry_residues(const double (&x)[L], int shift, const ResTab &t, int32_t *o,
+                                                   long long plane)
+{
+    constexpr int NC = (32 * NWU + 23) / 24;
+    uint32_t N[NWU];
+    to_fixed<L, NWU>(x, shift, N);
+    const bool neg = (int32_t)N[NWU - 1] < 0;
+    if (neg) {
+        uint32_t cy = 1;
+#pragma unroll
+        for (int j = 0; j < NWU; ++j) {
+            const uint32_t v = ~N[j] + cy;
+            cy = (cy && v == 0u) ? 1u : 0u;
+            N[j] = v;
+        }
+    }
+    const double sg = neg ? -1.0 : 1.0;
+    double ch[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ch[c] = sg * chunk24<NWU>(N, c);
+    for (int i = 0; i < t.nf; ++i) {
+        const double *pw = t.pw24 + i * RES_NC24;
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v = fma(ch[c], pw[c], v);
+        o[i * plane] = (int32_t)mod_bal_odd(v, t.md[i], t.inv[i]);
+    }
+}
+
+// A: thread per entry (r, kk) of the padded mp x kp domain; planes
+// R[i][r][kk] (mp x kp)
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void f64_residue_a_kernel(int m, int k, int mp, int kp, const double *__restrict__ A,
+                                                            long long lda, const int *__restrict__ E, int W, ResTab t,
+                                                            int32_t *__restrict__ R)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)mp * kp) return;
+    const long long r = idx / kp;
+    const int kk = (int)(idx - r * kp);
+    const bool live = r < m && kk < k && E[r] != ZERO_SCALE;
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = live ? A[(r * lda + kk) * L + l] : 0.0;
+    f64_entry_residues<L, NWU>(x, live ? W - 2 - E[r] : 0, t, R + r * kp + kk, (long long)mp * kp);
+}
+
+// B: a block stages a RES_TK x 4 tile of B in LDS (coalesced along the
+// columns), then each wave takes one column and 64 consecutive kk, so the
+// transposed planes R[i][c][kk] (np x kp) are written 256 bytes at a time.
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void f64_residue_b_kernel(int k, int n, int kp, int np, const double *__restrict__ B,
+                                                            long long ldb, const int *__restrict__ F, int W, ResTab t,
+                                                            int32_t *__restrict__ R)
+{
+    constexpr int TC = 4, ROW = TC * L + 1;
+    __shared__ double tile[RES_TK * ROW];
+    const int ktiles = kp / RES_TK;
+    const int c0 = (int)(blockIdx.x / ktiles) * TC;
+    const int k0 = (int)(blockIdx.x % ktiles) * RES_TK;
+    for (int q = threadIdx.x; q < RES_TK * TC * L; q += 256) {
+        const int row = q / (TC * L), col = q - row * (TC * L);
+        const int kk = k0 + row, c = c0 + col / L;
+        tile[row * ROW + col] = (kk < k && c < n) ? B[((long long)kk * ldb + c0) * L + col] : 0.0;
+    }
+    __syncthreads();
+    const int cc = threadIdx.x >> 6, kq = threadIdx.x & 63;
+    const int c = c0 + cc, kk = k0 + kq;
+    const bool live = c < n && F[c] != ZERO_SCALE;
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = live ? tile[kq * ROW + cc * L + l] : 0.0;
+    f64_entry_residues<L, NWU>(x, live ? W - 2 - F[c] : 0, t, R + (long long)c * kp + kk, (long long)np * kp);
+}
+
 // C[r][c] from the INT32 planes: narrow D8[i][q] (chunk q of nck1), wide
 // Dhh[j][q], Dll[j][q] (nck1 chunks) and Dx[j][q] (nck2 chunks); plane p of a
 // block at p * ps.  Two-level CRT in exact FP64:
@@ -295,17 +389,28 @@ __device__ __forceinline__ void crt_acc(double (&acc)[NW], double &tq, const Res
     for (int w = 0; w < NW; ++w) acc[w] = fma(R, cw[w], acc[w]);
 }
 
-template <int L, int NW>
-__global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, int W, ResTab t, int nck1, int nck2,
-                                                                  long long ps, int reduce,
-                                                                  const int *__restrict__ D8,
-                                                                  const int *__restrict__ Dhh,
-                                                                  const int *__restrict__ Dll,
-                                                                  const int *__restrict__ Dx,
+// the GEMM result planes of one call
+struct ResPlanes {
+    const int *D8, *Dhh, *Dll, *Dx;   // INT8 moduli: INT32 planes
+    const long long *D64;             // FP64 moduli: INT64 planes
+    long long ps;                     // plane stride (elements)
+    int nck1, nck2;                   // K-chunks of the depth-k and depth-2k GEMMs
+    int reduce;                       // narrow triples: reduce each plane value first
+};
+
+template <int L, int NW, bool F64>
+__global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, int W, ResTab t, ResPlanes pl,
                                                                   const int *__restrict__ E,
                                                                   const int *__restrict__ F, double *__restrict__ C,
                                                                   long long ldc)
 {
+    const int *__restrict__ D8 = pl.D8;
+    const int *__restrict__ Dhh = pl.Dhh;
+    const int *__restrict__ Dll = pl.Dll;
+    const int *__restrict__ Dx = pl.Dx;
+    const long long *__restrict__ D64 = pl.D64;
+    const long long ps = pl.ps;
+    const int nck1 = pl.nck1, nck2 = pl.nck2, reduce = pl.reduce;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)m * n) return;
     const long long r = idx / n, c = idx - r * n;
@@ -318,6 +423,23 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
 #pragma unroll
         for (int w = 0; w < NW; ++w) acc[w] = 0.0;
         double tq = 0.0;
+        if constexpr (F64) {
+            // INT64 planes of the FP64 moduli, |v| <= k 2^40: v = vh 2^32 + vl,
+            // v mod m = ((vh mod m) (2^32 mod m) + vl) mod m, all exact in FP64
+            for (int i0 = 0; i0 < t.nf; i0 += 2 * RES_GB) {
+                long long v[2 * RES_GB];
+#pragma unroll
+                for (int u = 0; u < 2 * RES_GB; ++u) v[u] = (i0 + u < t.nf) ? D64[(long long)(i0 + u) * ps + idx] : 0;
+#pragma unroll
+                for (int u = 0; u < 2 * RES_GB; ++u) {
+                    const int i = i0 + u;
+                    if (i >= t.nf) break;
+                    const double r1 = mod_bal_odd((double)(v[u] >> 32), t.md[i], t.inv[i]);
+                    const double lo = (double)(uint32_t)(v[u] & 0xFFFFFFFFll);
+                    crt_acc<NW>(acc, tq, t, i, mod_pos(fma(r1, t.c32[i], lo), t.md[i], t.inv[i]));
+                }
+            }
+        }
         for (int g0 = 0; g0 < t.ng8; g0 += RES_GB) {
             int v[RES_GB][3];
 #pragma unroll
@@ -512,11 +634,44 @@ static long long inv_mod(long long a, long long m)
     return ((x % m) + m) % m;
 }
 
-static std::vector<int> moduli_of(int n8, int n16)
+// FP64-MFMA moduli: odd primes below 2^21 from the top (balanced residues
+// below 2^20 in magnitude, products below 2^40)
+static const std::vector<int> &f64_moduli()
 {
+    static const std::vector<int> v = [] {
+        std::vector<int> out;
+        for (int p = (1 << 21) - 1; p > 3 && out.size() < 160; p -= 2) {
+            bool prime = true;
+            for (int d = 3; d * d <= p; d += 2)
+                if (p % d == 0) {
+                    prime = false;
+                    break;
+                }
+            if (prime) out.push_back(p);
+        }
+        return out;
+    }();
+    return v;
+}
+
+static std::vector<int> moduli_of(int n8, int n16, int nf = 0)
+{
+    if (nf > 0) return std::vector<int>(f64_moduli().begin(), f64_moduli().begin() + nf);
     std::vector<int> v(narrow_moduli().begin(), narrow_moduli().begin() + n8);
     v.insert(v.end(), wide_moduli().begin(), wide_moduli().begin() + n16);
     return v;
+}
+
+// Fewest FP64 moduli with M >= 2^need; *bits = bit length of M.
+static bool choose_f64_moduli(int need, int &nf, int *bits)
+{
+    Big M{1u};
+    for (nf = 0; nf < (int)f64_moduli().size();) {
+        big_mul_small(M, (uint32_t)f64_moduli()[nf++]);
+        *bits = big_bitlen(M);
+        if (*bits >= need + 1) return true;
+    }
+    return false;
 }
 
 // Cheapest set (n8 + 4 n16 GEMM units) with M >= 2^need; *bits = bit length of M.
@@ -565,7 +720,7 @@ struct ResTabHost {
 };
 
 static std::mutex g_res_mu;
-static std::map<std::tuple<int, int, int, int>, ResTabHost *> g_res;
+static std::map<std::tuple<int, int, int, int, int>, ResTabHost *> g_res;
 
 static double balanced(long long p, long long m) { return (double)(2 * p >= m ? p - m : p); }
 
@@ -586,23 +741,25 @@ static bool big_words24(const Big &b, int nw, double *out)
 }
 
 // device tables of a modulus set with CRT constants of nw 24-bit words (cached)
-static ResTabHost *res_table(int n8, int n16, int nw)
+static ResTabHost *res_table(int n8, int n16, int nf, int nw)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(g_res_mu);
-    const auto key = std::make_tuple(dev, n8, n16, nw);
+    const auto key = std::make_tuple(dev, n8, n16, nf, nw);
     auto it = g_res.find(key);
     if (it != g_res.end()) return it->second;
 
-    const std::vector<int> mods = moduli_of(n8, n16);
+    const std::vector<int> mods = moduli_of(n8, n16, nf);
     const int n = (int)mods.size();
-    const int ng8 = (n8 + 2) / 3, G = ng8 + n16;
-    // the groups: narrow triples (3g, 3g+1, 3g+2), then each wide modulus
+    const int ng8 = (n8 + 2) / 3, G = ng8 + n16 + nf;
+    // the groups: narrow triples (3g, 3g+1, 3g+2), then each wide modulus;
+    // or each FP64 modulus
     std::vector<long long> P(G);
     std::vector<std::vector<int>> members(G);
     for (int i = 0; i < n8; ++i) members[i / 3].push_back(i);
     for (int j = 0; j < n16; ++j) members[ng8 + j].push_back(n8 + j);
+    for (int i = 0; i < nf; ++i) members[i].push_back(i);
     for (int g = 0; g < G; ++g) {
         P[g] = 1;
         for (int i : members[g]) P[g] *= mods[i];
@@ -610,7 +767,7 @@ static ResTabHost *res_table(int n8, int n16, int nw)
     const size_t o_inv = n, o_p37 = 2 * (size_t)n, o_p37h = o_p37 + (size_t)n8 * RES_NC,
                  o_p37l = o_p37h + (size_t)n16 * RES_NC, o_gP = o_p37l + (size_t)n16 * RES_NC, o_ginv = o_gP + G,
                  o_gfrac = o_ginv + G, o_gE = o_gfrac + G, o_cw = o_gE + 3 * (size_t)ng8, o_M = o_cw + (size_t)G * nw,
-                 total = o_M + nw;
+                 o_p24 = o_M + nw, o_c32 = o_p24 + (size_t)nf * RES_NC24, total = o_c32 + nf;
     std::vector<double> h(total, 0.0);
     for (int i = 0; i < n; ++i) {
         h[i] = (double)mods[i];
@@ -633,6 +790,15 @@ static ResTabHost *res_table(int n8, int n16, int nw)
             h[o_p37l + (size_t)j * RES_NC + c] = (double)lo;
             p = (long long)(((unsigned __int128)p << 37) % (uint64_t)mj);
         }
+    }
+    for (int i = 0; i < nf; ++i) {
+        const long long mi = mods[i];
+        long long p = 1;
+        for (int c = 0; c < RES_NC24; ++c) {
+            h[o_p24 + (size_t)i * RES_NC24 + c] = balanced(p, mi);
+            p = (p << 24) % mi;   // p < 2^21: no overflow
+        }
+        h[o_c32 + i] = balanced((1ll << 32) % mi, mi);
     }
     Big M{1u};
     for (int m : mods) big_mul_small(M, (uint32_t)m);
@@ -681,6 +847,9 @@ static ResTabHost *res_table(int n8, int n16, int nw)
     r->t.gE = d + o_gE;
     r->t.cw = d + o_cw;
     r->t.Mw = d + o_M;
+    r->t.nf = nf;
+    r->t.pw24 = d + o_p24;
+    r->t.c32 = d + o_c32;
     g_res[key] = r;
     return r;
 }
@@ -748,24 +917,24 @@ static int residues_dispatch(int nwu, int m, int n, int k, int mp, int np, int k
 }
 
 template <int L, int NW>
-static void launch_reconstruct(int m, int n, int W, const ResTab &t, int nck1, int nck2, long long ps, int reduce,
-                               const int *D8,
-                               const int *Dhh, const int *Dll, const int *Dx, const int *E, const int *F, double *C,
-                               long long ldc, hipStream_t s)
+static void launch_reconstruct(bool f64, int m, int n, int W, const ResTab &t, const ResPlanes &pl, const int *E,
+                               const int *F, double *C, long long ldc, hipStream_t s)
 {
-    hipLaunchKernelGGL((residue_reconstruct_kernel<L, NW>), dim3(grid_for((long long)m * n, 256)), dim3(256), 0, s,
-                       m, n, W, t, nck1, nck2, ps, reduce, D8, Dhh, Dll, Dx, E, F, C, ldc);
+    if (f64)
+        hipLaunchKernelGGL((residue_reconstruct_kernel<L, NW, true>), dim3(grid_for((long long)m * n, 256)),
+                           dim3(256), 0, s, m, n, W, t, pl, E, F, C, ldc);
+    else
+        hipLaunchKernelGGL((residue_reconstruct_kernel<L, NW, false>), dim3(grid_for((long long)m * n, 256)),
+                           dim3(256), 0, s, m, n, W, t, pl, E, F, C, ldc);
 }
 
 template <int L>
-static int reconstruct_dispatch(int nw, int m, int n, int W, const ResTab &t, int nck1, int nck2, long long ps,
-                                int reduce, const int *D8,
-                                const int *Dhh, const int *Dll, const int *Dx, const int *E, const int *F,
-                                double *C, long long ldc, hipStream_t s)
+static int reconstruct_dispatch(bool f64, int nw, int m, int n, int W, const ResTab &t, const ResPlanes &pl,
+                                const int *E, const int *F, double *C, long long ldc, hipStream_t s)
 {
-#define MDG_REC_CASE(NW)                                                                               \
-    case NW:                                                                                           \
-        launch_reconstruct<L, NW>(m, n, W, t, nck1, nck2, ps, reduce, D8, Dhh, Dll, Dx, E, F, C, ldc, s); \
+#define MDG_REC_CASE(NW)                                                 \
+    case NW:                                                             \
+        launch_reconstruct<L, NW>(f64, m, n, W, t, pl, E, F, C, ldc, s); \
         return MDG_SUCCESS;
     if constexpr (L == 2) {
         switch (nw) { MDG_REC_CASE(12) MDG_REC_CASE(16) MDG_REC_CASE(20) MDG_REC_CASE(24) }
@@ -828,7 +997,7 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
     MDG_REQUIRE(choose_moduli(need, n8, n16, &mbits), MDG_ERROR_RANGE, "no modulus set reaches %d bits", need);
     const int nwu = nwu_bucket<L>(W);
     const int nw = nw_bucket(L, mbits);
-    ResTabHost *th = res_table(n8, n16, nw);
+    ResTabHost *th = res_table(n8, n16, 0, nw);
     MDG_REQUIRE(th, MDG_ERROR_HIP, "could not build the modulus tables (%d + %d moduli)", n8, n16);
     const ResTab &t = th->t;
 
@@ -869,8 +1038,8 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
         }
     }
     tm.mark(3);
-    rc = reconstruct_dispatch<L>(nw, m, n, W, t, nck1, nck2, ps, (nck1 > 1 || kp > 8192) ? 1 : 0, D8, Dhh, Dll, Dx,
-                                 E, F, C, ldc, s);
+    const ResPlanes pl{D8, Dhh, Dll, Dx, nullptr, ps, nck1, nck2, (nck1 > 1 || kp > 8192) ? 1 : 0};
+    rc = reconstruct_dispatch<L>(false, nw, m, n, W, t, pl, E, F, C, ldc, s);
     if (rc) return rc;
     MDG_LAUNCH_CHECK("residue_reconstruct_kernel");
     tm.mark(4);
@@ -892,10 +1061,133 @@ static int run_residue_gemm(int m, int n, int k, const double *A, long long lda,
     return MDG_SUCCESS;
 }
 
+// ------------------------------------------------ FP64-MFMA residue GEMM --
+template <int L, int NWU>
+static void launch_f64_residues(int m, int n, int k, int mp, int np, int kp, const double *A, long long lda,
+                                const double *B, long long ldb, const int *E, const int *F, int W, const ResTab &t,
+                                int32_t *RA, int32_t *RB, hipStream_t s)
+{
+    hipLaunchKernelGGL((f64_residue_a_kernel<L, NWU>), dim3(grid_for((long long)mp * kp, 256)), dim3(256), 0, s, m,
+                       k, mp, kp, A, lda, E, W, t, RA);
+    hipLaunchKernelGGL((f64_residue_b_kernel<L, NWU>), dim3((unsigned)((np / 4) * (kp / RES_TK))), dim3(256), 0, s,
+                       k, n, kp, np, B, ldb, F, W, t, RB);
+}
+
+template <int L>
+static int f64_residues_dispatch(int nwu, int m, int n, int k, int mp, int np, int kp, const double *A,
+                                 long long lda, const double *B, long long ldb, const int *E, const int *F, int W,
+                                 const ResTab &t, int32_t *RA, int32_t *RB, hipStream_t s)
+{
+#define MDG_RES_CASE(NWU)                                                                               \
+    case NWU:                                                                                           \
+        if constexpr (NWU <= FixedWords<L>::NW) {                                                       \
+            launch_f64_residues<L, NWU>(m, n, k, mp, np, kp, A, lda, B, ldb, E, F, W, t, RA, RB, s); \
+            return MDG_SUCCESS;                                                                         \
+        }                                                                                               \
+        break;
+    switch (nwu) {
+        MDG_RES_CASE(4)
+        MDG_RES_CASE(8)
+        MDG_RES_CASE(12)
+        MDG_RES_CASE(16)
+        MDG_RES_CASE(20)
+        MDG_RES_CASE(24)
+        MDG_RES_CASE(28)
+        MDG_RES_CASE(32)
+        MDG_RES_CASE(36)
+    }
+#undef MDG_RES_CASE
+    set_error("no residue kernel for %d fixed-point words", nwu);
+    return MDG_ERROR_RANGE;
+}
+
+// Residues modulo odd primes below 2^21, one INT32 plane per modulus and
+// operand, products on the FP64 MFMA kernel (exact FP64 partial sums moved
+// into INT64 every 8192 k: 8192 products of magnitude below 2^40 stay below
+// 2^53), then the same CRT reconstruction with one group per modulus.
+template <int L>
+static int run_residue_f64(int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                           double *C, long long ldc, int target_bits, int fixed_spread, mdg_gemm_info *info,
+                           hipStream_t s)
+{
+    StageTimer tm(s, profiling_on());
+    tm.mark(0);
+    MDG_REQUIRE(k <= (1 << 22), MDG_ERROR_RANGE, "the residue GEMM supports k <= 2^22 (k=%d)", k);
+    int *E = nullptr, *F = nullptr;
+    int spread_a = 0, spread_b = 0;
+    int rc = scan_scales(L, m, n, k, A, lda, B, ldb, fixed_spread, s, &E, &F, &spread_a, &spread_b);
+    if (rc) return rc;
+    tm.mark(1);
+
+    int lgL = 0;
+    while ((1 << lgL) < L) ++lgL;
+    int lgk = 0;
+    while ((1ll << lgk) < k) ++lgk;
+    const int W = target_bits + spread_a + spread_b + 3 + lgL;
+    MDG_REQUIRE(W <= 32 * FixedWords<L>::NW - 1, MDG_ERROR_RANGE,
+                "needs %d fixed-point bits, more than the %d this limb count holds", W, 32 * FixedWords<L>::NW - 1);
+    const int need = 2 * W - 1 + lgk;
+    int nf = 0, mbits = 0;
+    MDG_REQUIRE(choose_f64_moduli(need, nf, &mbits), MDG_ERROR_RANGE, "no FP64 modulus set reaches %d bits", need);
+    const int nwu = nwu_bucket<L>(W);
+    const int nw = nw_bucket(L, mbits);
+    ResTabHost *th = res_table(0, 0, nf, nw);
+    MDG_REQUIRE(th, MDG_ERROR_HIP, "could not build the modulus tables (%d FP64 moduli)", nf);
+    const ResTab &t = th->t;
+
+    const int kp = (int)round_up(k, 128), mp = (int)round_up(m, 128), np = (int)round_up(n, 128);
+    const long long ps = plane_stride((long long)m * n, 8);
+    int32_t *RA = (int32_t *)scratch(s, (size_t)nf * mp * kp * 4 + 256, 2);
+    int32_t *RB = (int32_t *)scratch(s, (size_t)nf * np * kp * 4 + 256, 3);
+    long long *D = (long long *)scratch(s, (size_t)nf * ps * 8, 4);
+    MDG_REQUIRE(RA && RB && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    rc = f64_residues_dispatch<L>(nwu, m, n, k, mp, np, kp, A, lda, B, ldb, E, F, W, t, RA, RB, s);
+    if (rc) return rc;
+    MDG_LAUNCH_CHECK("FP64 residue kernels");
+    tm.mark(2);
+    for (int i = 0; i < nf; ++i) {
+        rc = f64_gemm_i32(RA + (size_t)i * mp * kp, kp, RB + (size_t)i * np * kp, kp, m, n, kp, 8192, D + i * ps, s);
+        if (rc) return rc;
+    }
+    tm.mark(3);
+    const ResPlanes pl{nullptr, nullptr, nullptr, nullptr, D, ps, 1, 1, 0};
+    rc = reconstruct_dispatch<L>(true, nw, m, n, W, t, pl, E, F, C, ldc, s);
+    if (rc) return rc;
+    MDG_LAUNCH_CHECK("residue_reconstruct_kernel");
+    tm.mark(4);
+    float stage[4] = {0.f, 0.f, 0.f, 0.f};
+    if (tm.on) MDG_HIP_CHECK(tm.elapsed(stage));
+    if (info) {
+        info->method = MDG_METHOD_F64_RESIDUE;
+        info->slices = nf;
+        info->slice_bits = 21;
+        info->diagonals = 0;
+        info->pair_gemms = nf;
+        info->total_bits = W;
+        info->spread_a = spread_a;
+        info->spread_b = spread_b;
+        info->flushed = 1;
+        info->backend = 2;
+        for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
+    }
+    return MDG_SUCCESS;
+}
+
 int residue_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
-                 double *C, long long ldc, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s)
+                 double *C, long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info,
+                 hipStream_t s)
 {
     if (target_bits <= 0) target_bits = 53 * L;
+    if (method == MDG_METHOD_F64_RESIDUE) {
+        switch (L) {
+        case 2: return run_residue_f64<2>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
+        case 4: return run_residue_f64<4>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
+        case 8: return run_residue_f64<8>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
+        case 16: return run_residue_f64<16>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
+        }
+        set_error("unsupported limb count L=%d", L);
+        return MDG_ERROR_INVALID_VALUE;
+    }
     switch (L) {
     case 2: return run_residue_gemm<2>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
     case 4: return run_residue_gemm<4>(m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);

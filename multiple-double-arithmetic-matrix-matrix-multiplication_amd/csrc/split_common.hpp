@@ -34,6 +34,12 @@ int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long l
 int scan_scales(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
                 int fixed_spread, hipStream_t s, int **E, int **F, int *spread_a, int *spread_b);
 
+// D (m x n INT64, row stride n) = A (m x depth, row stride lda) * Bt^T
+// (Bt: n x depth, row stride ldb) on the FP64 MFMA kernel, INT32 operands
+// with |a b| * flushk <= 2^53 so every FP64 partial sum is exact.
+int f64_gemm_i32(const int32_t *A, long long lda, const int32_t *Bt, long long ldb, int m, int n, int depth, int flushk,
+                 long long *D, hipStream_t s);
+
 inline long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
 
 inline unsigned grid_for(long long nel, int bs) { return (unsigned)((nel + bs - 1) / bs); }

@@ -1047,13 +1047,14 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
 }
 
 int residue_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
-                 double *C, long long ldc, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s);
+                 double *C, long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info,
+                 hipStream_t s);
 
 int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
                long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s)
 {
-    if (method == MDG_METHOD_INT8_RESIDUE)
-        return residue_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, target_bits, fixed_spread, info, s);
+    if (method == MDG_METHOD_INT8_RESIDUE || method == MDG_METHOD_F64_RESIDUE)
+        return residue_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, fixed_spread, info, s);
     Plan p{};
     p.method = method;
     p.L = L;
@@ -1074,6 +1075,18 @@ int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const
 // ------------------------------------------- reference split-path pieces --
 // The generic FP64 TN kernel serves the reference's accumulate GEMM
 // (C++/TCtest.cu:21-46) and the limb convolution (Aq * bigB2(Bq)).
+
+int f64_gemm_i32(const int32_t *A, long long lda, const int32_t *Bt, long long ldb, int m, int n, int depth, int flushk,
+                 long long *D, hipStream_t s)
+{
+    const int tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
+    dim3 grid(tiles_m * tiles_n, 1, 1);
+    // one "diagonal" (S = 1) of the given depth, INT64 output plane of row stride n
+    hipLaunchKernelGGL((diag_f64_kernel<int32_t, 2>), grid, dim3(512), 0, s, A, Bt, 1, depth, lda, ldb, m, n, tiles_m,
+                       tiles_n, flushk, (void *)D, (long long)n);
+    MDG_LAUNCH_CHECK("diag_f64_kernel (residue plane)");
+    return MDG_SUCCESS;
+}
 
 int dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, hipStream_t s)
 {

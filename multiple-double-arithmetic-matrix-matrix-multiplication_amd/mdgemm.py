@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libmdgemm.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mdgemm.h")
 
 METHOD_VECTOR, METHOD_F64_SPLIT, METHOD_INT8_OZAKI, METHOD_BF16_OZAKI = 0, 1, 2, 3
-METHODS = {"vector": 0, "f64": 1, "int8": 2, "bf16": 3, "int8_residue": 4}
+METHODS = {"vector": 0, "f64": 1, "int8": 2, "bf16": 3, "int8_residue": 4, "f64_residue": 5}
 TWOPROD_FMA, TWOPROD_DEKKER = 0, 1
 GEN_DD, GEN_QD, GEN_REF = 0, 1, 2
 

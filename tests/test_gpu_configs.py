@@ -40,7 +40,7 @@ def test_config1_dd_n128(mdg, orc, torch_cuda):
     got = to_host(mdg.vector_gemm(L, n, n, n, A, B, twoprod=mdg.TWOPROD_DEKKER))
     assert_bits_equal(got, want, "dd vector GEMM vs host GEMM")
     assert_bits_equal(to_host(mdg.vector_gemm(L, n, n, n, A, B)), want, "fma two_prod")
-    for method in ("int8", "f64", "bf16", "int8_residue"):
+    for method in ("int8", "f64", "bf16", "int8_residue", "f64_residue"):
         C, info = mdg.gemm(L, n, n, n, A, B, method=method)
         e = _err(orc, L, n, n, Ah, Bh, to_host(C), n * n)
         print(f"cfg1 dd n=128 {method}: err {e:.3e} S={info['slices']}")
@@ -55,7 +55,8 @@ def test_config2_qd_n1024_f64_split_vs_vector(mdg, orc, torch_cuda):
     Cv, tv = _timed(torch_cuda, lambda: mdg.vector_gemm(L, n, n, n, A, B))
     (Cs, info), ts = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method="f64"))
     (Cr, _), tr = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method="int8_residue"))
-    Cv, Cs, Cr = to_host(Cv), to_host(Cs), to_host(Cr)
+    (Cf, _), tf = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method="f64_residue"))
+    Cv, Cs, Cr, Cf = to_host(Cv), to_host(Cs), to_host(Cr), to_host(Cf)
     # vector GEMM bit-identical to the host loops on 3 sampled rows
     for r in (0, 517, 1023):
         want = orc.gemm_rows(L, r, r + 1, n, n, Ah, Bh, mode=orc.TP_FMA, nthreads=8)
@@ -64,10 +65,12 @@ def test_config2_qd_n1024_f64_split_vs_vector(mdg, orc, torch_cuda):
     es = _err(orc, L, n, n, Ah, Bh, Cs, 1024)
     er = _err(orc, L, n, n, Ah, Bh, Cr, 1024)
     print(f"cfg2 qd n=1024: vector {tv:.1f} ms err {ev:.3e}; f64 split {ts:.1f} ms err {es:.3e} "
-          f"S={info['slices']} w={info['slice_bits']}; int8 residues {tr:.1f} ms err {er:.3e}")
+          f"S={info['slices']} w={info['slice_bits']}; int8 residues {tr:.1f} ms err {er:.3e}; "
+          f"f64 residues {tf:.1f} ms")
     assert ev <= _bound(L, n) and es <= _bound(L, n) and er <= _bound(L, n)
     # the split paths are at least as accurate as the reference arithmetic
     assert es <= ev and er <= ev
+    assert np.array_equal(Cr.view(np.uint64), Cf.view(np.uint64))   # same exact product
 
 
 def test_config3_od_2048_int8_bf16_vs_f64(mdg, orc, torch_cuda):
@@ -76,15 +79,17 @@ def test_config3_od_2048_int8_bf16_vs_f64(mdg, orc, torch_cuda):
     B = mdg.fill_random(mdg.GEN_QD, L, n, n, 302, -30, 30)
     Ah, Bh = to_host(A), to_host(B)
     res = {}
-    for method in ("f64", "int8", "bf16", "int8_residue"):
+    for method in ("f64", "int8", "bf16", "int8_residue", "f64_residue"):
         (C, info), t = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method=method))
         e = _err(orc, L, n, n, Ah, Bh, to_host(C), 4096)
         res[method] = to_host(C)
         print(f"cfg3 od n=2048 {method}: {t:.1f} ms err {e:.3e} S={info['slices']} "
               f"w={info['slice_bits']} P={info['pair_gemms']}")
         assert e <= _bound(L, n)
-    # INT8 and BF16 slice identically and sum exactly: identical bits
+    # INT8 and BF16 slice identically and sum exactly: identical bits; so do
+    # the two residue methods
     assert_bits_equal(res["int8"], res["bf16"], "int8 vs bf16")
+    assert_bits_equal(res["int8_residue"], res["f64_residue"], "int8 vs f64 residues")
 
 
 def test_config4_batched_1024_mixed(mdg, orc, torch_cuda):
@@ -122,7 +127,7 @@ def hd4096(mdg, torch_cuda):
     return L, n, A, B, to_host(A), to_host(B)
 
 
-@pytest.mark.parametrize("method", ["int8", "f64", "int8_residue"])
+@pytest.mark.parametrize("method", ["int8", "f64", "int8_residue", "f64_residue"])
 def test_config5_hd_4096_split(mdg, orc, torch_cuda, hd4096, method):
     L, n, A, B, Ah, Bh = hd4096
     (C, info), t = _timed(torch_cuda, lambda: mdg.gemm(L, n, n, n, A, B, method=method))

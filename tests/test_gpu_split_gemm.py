@@ -9,7 +9,7 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "int8_hipblaslt", "bf16", "int8_residue"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "int8_hipblaslt", "bf16", "int8_residue", "f64_residue"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
@@ -46,6 +46,8 @@ def test_split_gemm_error_bound(mdg, orc, torch_cuda, method, L):
         if method == "int8_residue":
             # narrow moduli: one INT8 GEMM each, wide moduli: four
             assert info["pair_gemms"] == (S - info["diagonals"]) + 4 * info["diagonals"]
+        elif method == "f64_residue":
+            assert info["pair_gemms"] == S and info["slice_bits"] == 21
         else:
             assert S >= 1 and info["pair_gemms"] == S * (S + 1) // 2
         V = orc.gemm(L, m, n, k, A, B, mode=orc.TP_FMA)
@@ -144,6 +146,16 @@ def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
         assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
 
 
+def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):
+    # k > 8192: the FP64 partial sums are moved into INT64 more than once
+    L, m, n, k = 4, 40, 36, 20000
+    A = orc.generate(orc.GEN_QD, L, m, k, 25, -6, 6)
+    B = orc.generate(orc.GEN_QD, L, k, n, 26, -6, 6)
+    C, info = mdg.gemm(L, m, n, k, to_dev(A), to_dev(B), method="f64_residue")
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=128)
+    assert err <= _bound(L, k), (err, info)
+
+
 def test_residue_chunked_depth(mdg, orc, torch_cuda):
     # 2 kp > 130944: the depth-2k GEMMs of the wide moduli are cut into
     # K-chunks whose INT32 planes are added in INT64 before the reduction
@@ -173,3 +185,17 @@ def test_residue_engines_agree_bitwise(mdg, orc, torch_cuda, L):
         assert info["backend"] == (engine or info["backend"]) and info["backend"] in (1, 3)
         out[engine] = to_host(C).view(np.uint64)
     assert np.array_equal(out[1], out[3]) and np.array_equal(out[1], out[0])
+
+
+@pytest.mark.parametrize("L", [2, 4, 8, 16])
+def test_int8_and_f64_residues_bit_identical(mdg, orc, torch_cuda, L):
+    # both residue methods compute the same exact integer product of the same
+    # fixed-point inputs (different moduli, different MFMA formats), so the
+    # reconstructed results must agree bit for bit
+    m, n, k = 70, 90, 110
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 91 + L, -25, 25))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 92 + L, -25, 25))
+    C8, i8 = mdg.gemm(L, m, n, k, A, B, method="int8_residue")
+    C64, i64 = mdg.gemm(L, m, n, k, A, B, method="f64_residue")
+    assert i8["total_bits"] == i64["total_bits"]
+    assert np.array_equal(to_host(C8).view(np.uint64), to_host(C64).view(np.uint64))
