@@ -7,9 +7,11 @@
 // along the depth, i.e. D_b is the row-major m x n product of A_b (m x depth)
 // and Bt_b^T.  Sums are exact INT32 as long as depth * 2^14 < 2^31.
 //
-// The first call of a shape times every algorithm hipBLASLt proposes for it
-// and rocBLAS's default, twice each, and keeps the fastest; all of them
-// compute the same exact integers.
+// The first call of a shape bucket (power-of-two buckets of batch, m, n and
+// depth) times the first algorithms hipBLASLt proposes for it and rocBLAS's
+// default, twice each, and keeps the fastest; all of them compute the same
+// exact integers.  A kept hipBLASLt algorithm that rejects another shape of
+// its bucket falls back to rocBLAS for that call.
 #include <hipblaslt/hipblaslt.h>
 
 #include <algorithm>
@@ -50,7 +52,16 @@ LtHandle *lt_handle()
     return l;
 }
 
-typedef std::tuple<int, int, int, int, int, long long, long long, long long, long long, long long, long long> ShapeKey;
+typedef std::tuple<int, int, int, int, int, int> ShapeKey;
+
+constexpr int LT_CANDIDATES = 8;
+
+int lg2_ceil(long long v)
+{
+    int b = 0;
+    while ((1ll << b) < v) ++b;
+    return b;
+}
 
 struct Choice {
     int engine = 0;   // 1 rocBLAS, 3 hipBLASLt
@@ -105,7 +116,7 @@ int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long l
     int dev = 0;
     MDG_HIP_CHECK(hipGetDevice(&dev));
     const int forced = int8_backend();   // 0 autotuned, 1 rocBLAS, 3 hipBLASLt
-    const ShapeKey key{dev, forced, batch, m, n, (long long)depth, ldb, sb, lda, sa, sd};
+    const ShapeKey key{dev, forced, lg2_ceil(batch), lg2_ceil(m), lg2_ceil(n), lg2_ceil(depth)};
 
     BlasHandle *bh = blas_handle();
     MDG_REQUIRE(bh, MDG_ERROR_HIP, "rocblas_create_handle failed");
@@ -162,8 +173,12 @@ int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long l
     LtDescs d(batch, m, n, depth, ldb, sb, lda, sa, ldd, sd);
     MDG_REQUIRE(d.ok, MDG_ERROR_HIP, "hipBLASLt descriptor set-up failed");
     if (known) {
-        if (engine_used) *engine_used = 3;
-        return run_lt(d, ch.algo);
+        if (run_lt(d, ch.algo) == MDG_SUCCESS) {
+            if (engine_used) *engine_used = 3;
+            return MDG_SUCCESS;
+        }
+        if (engine_used) *engine_used = 1;
+        return run_rocblas();
     }
 
     // autotune: hipBLASLt's candidates (and rocBLAS unless hipBLASLt is forced)
@@ -172,7 +187,7 @@ int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long l
                 "hipblasLtMatmulPreferenceCreate failed");
     const uint64_t wmax = LT_WORKSPACE;
     (void)hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wmax, sizeof(wmax));
-    std::vector<hipblasLtMatmulHeuristicResult_t> res(16);
+    std::vector<hipblasLtMatmulHeuristicResult_t> res(LT_CANDIDATES);
     int got = 0;
     {
         std::lock_guard<std::mutex> lock(lh->mu);
