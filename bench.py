@@ -29,6 +29,12 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 matrix / vector
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 
 
+# FP64 operations per multi-double multiply-add of the reference's vector
+# kernels (C++/TCtest.cu:137-147; the hexa-double figure counted from
+# C++/hexa_double_functions.cu)
+VECTOR_OPS_PER_MAC = {2: 43, 4: 425, 8: 2011, 16: 11800}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +118,15 @@ def main():
         ms = (time.perf_counter() - t0) * 1e3 / steps
         stage /= steps
         outputs[path] = C.cpu().numpy()
+        # work per GEMM: effective 2mnk; matrix-core ops of the products the
+        # method issues (slice pairs, or residue GEMM units of depth k); bytes
+        # of the operands and the result
+        units = info["pair_gemms"] if path != "vector" else 0
+        work = {"w_eff": eff_flops, "w_mfma": units * eff_flops if units else None,
+                "w_valu": (VECTOR_OPS_PER_MAC[L] * m * n * k) if path == "vector" else None,
+                "bytes": (m * k + k * n + m * n) * L * 8}
         results[path] = {"ms": ms, "gflops_eff": eff_flops / ms / 1e6, "stage_ms": stage.tolist(),
+                         "work": work,
                          "slices": info["slices"], "slice_bits": info["slice_bits"],
                          "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
                          "spread": [info["spread_a"], info["spread_b"]], "steps": steps,
@@ -184,11 +198,30 @@ def main():
             rows += min(m, r0 + chunk) - r0
             r0 += chunk
         cpu_gflops = 2.0 * rows * n * k / t_cpu / 1e9
+        # the reference's own mode: one thread, a few rows
+        r1, t1 = 0, 0.0
+        while t1 < a.cpu_seconds / 4 and r1 < m:
+            t0 = time.perf_counter()
+            orc.gemm_rows(L, r1, r1 + 1, n, k, Ah, Bh, mode=orc.TP_DEKKER, nthreads=1)
+            t1 += time.perf_counter() - t0
+            r1 += 1
+        model = "unknown"
+        try:
+            for ln in open("/proc/cpuinfo"):
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": cpu_gflops, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
                "sample": f"{rows} of {m} output rows of the qd n={n} GEMM (host loops of "
                          f"C++/TCcuda.cu:334-346 restated, Dekker two_prod), {t_cpu:.1f} s, "
-                         f"scaled by effective 2*rows*n*k flops"}
-        log(f"cpu baseline: {cpu_gflops:.3f} GFLOP/s on {nthreads} threads ({rows} rows)")
+                         f"scaled by effective 2*rows*n*k flops",
+               "single_thread": {"value": 2.0 * r1 * n * k / t1 / 1e9, "unit": "GFLOP/s", "cores": 1,
+                                 "sample": f"{r1} output rows, {t1:.1f} s"},
+               "cpu_model": model, "host_threads": os.cpu_count()}
+        log(f"cpu baseline: {cpu_gflops:.3f} GFLOP/s on {nthreads} threads ({rows} rows); "
+            f"{cpu['single_thread']['value']:.4f} GFLOP/s on 1 thread ({r1} rows); {model}")
 
     def ratio(a_, b_):
         return results[a_]["gflops_eff"] / results[b_]["gflops_eff"] if a_ in results and b_ in results else None

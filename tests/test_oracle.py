@@ -99,6 +99,32 @@ def test_exact_checker_equals_rational_arithmetic(orc, L):
             assert Fraction(v) * Fraction(2) ** base == _frac_entry(L, k, A, B, n, r, c)
 
 
+def test_exact_checker_agrees_with_mpmath_1100_bits(orc):
+    # the normwise error is specified against an MPFR-class oracle at >= 1100
+    # bits; the checker is exact, so mpmath at 1100 bits must agree with it to
+    # that precision, and the per-entry error (exact - C) must round the same
+    mpmath = pytest.importorskip("mpmath")
+    L, m, n, k = 16, 3, 4, 7
+    A = orc.generate(orc.GEN_QD, L, m, k, 71, -60, 60)
+    B = orc.generate(orc.GEN_QD, L, k, n, 72, -60, 60)
+    C = orc.gemm(L, m, n, k, A, B)
+    rr, cc = np.divmod(np.arange(m * n), n)
+    err, _, _ = orc.exact_error(L, m, n, k, A, B, C, rr, cc)
+    with mpmath.workprec(1100):
+        for e in range(m * n):
+            r, c = int(rr[e]), int(cc[e])
+            s = mpmath.mpf(0)
+            for kk in range(k):
+                a = mpmath.fsum(mpmath.mpf(float(v)) for v in A[(r * k + kk) * L:(r * k + kk + 1) * L])
+                b = mpmath.fsum(mpmath.mpf(float(v)) for v in B[(kk * n + c) * L:(kk * n + c + 1) * L])
+                s += a * b
+            v, base = orc.exact_entry(L, k, A, k, B, n, r, c)
+            exact = mpmath.mpf(v) * mpmath.mpf(2) ** base
+            assert abs(exact - s) <= abs(exact) * mpmath.mpf(2) ** -1090
+            chat = mpmath.fsum(mpmath.mpf(float(x)) for x in C[e * L:(e + 1) * L])
+            assert float(s - chat) == err[e]
+
+
 def test_exact_checker_error_of_exact_result_is_zero(orc):
     # small integers: every product and sum is exact in one double
     rng = np.random.default_rng(5)
