@@ -20,10 +20,12 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
 #include "md_arith.hpp"
+#include "recon.hpp"
 #include "slices.hpp"
 
 namespace mdg {
@@ -268,7 +270,9 @@ __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict_
 }
 
 // ------------------------------------------------------- reconstruction --
-template <int L>
+// the same exact fold as the single-problem INT8 path (recon.hpp), so the
+// fused pipeline and the per-problem loop give identical bits
+template <int L, int NW, bool CHUNKED>
 __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
     const BEnt e = T[blockIdx.x];
@@ -282,22 +286,23 @@ __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restr
     const int er = p.E[r], fc = p.F[c];
     if (er != BZERO_SCALE && fc != BZERO_SCALE) {
         const long long mn = (long long)p.m * p.n;
-        int first = 0;
-        for (int g = 0; g < p.S; ++g) {
-            const int nc = b_chunks(g, p.kp, p.kmax);
-            long long v = 0;
-            for (int q = 0; q < nc; ++q) v += p.D[(long long)(first + q) * mn + idx];
-            first += nc;
-            if (v == 0) continue;
-            const int sc = er + fc + 4 - p.w * (g + 2);
-            if (v > -(1ll << 53) && v < (1ll << 53)) {
-                md_inc_d<L>(y, ldexp((double)v, sc));
-            } else {
-                const long long hi = v & ~0xFFFFFFFFll, lo = v - hi;
-                md_inc_d<L>(y, ldexp((double)hi, sc));
-                if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
-            }
-        }
+        diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
+                         [&](int g) {
+                             if constexpr (!CHUNKED) return (long long)p.D[(long long)g * mn + idx];
+                             int first;
+                             if (p.kmax % p.kp == 0) {
+                                 const int cc = p.kmax / p.kp, q = g / cc, rr = g - q * cc;
+                                 first = cc * q * (q + 1) / 2 + rr * (q + 1);
+                             } else {
+                                 first = 0;
+                                 for (int h = 0; h < g; ++h) first += b_chunks(h, p.kp, p.kmax);
+                             }
+                             const int nc = b_chunks(g, p.kp, p.kmax);
+                             long long v = 0;
+                             for (int q = 0; q < nc; ++q) v += p.D[(long long)(first + q) * mn + idx];
+                             return v;
+                         },
+                         y);
     }
     double *o = p.C + (r * p.ldc + c) * L;
 #pragma unroll
@@ -361,6 +366,8 @@ struct Stage {
     int kind;      // 0 init, 1 scan rows, 2 scan cols, 3 finish, 4 slice A, 5 slice B, 6 gemm, 7 recon, 8 vector
     int L;         // limb count of the stage's problems (0: any)
     long long first, count;   // range in the entry table
+    int nw = 0;    // reconstruction: diagonal-sum words its problems need
+    bool chunked = false;   // reconstruction: some problem has K-chunk planes
 };
 
 struct Plan {
@@ -510,11 +517,19 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
             alt.count = (long long)pl.ents.size() - alt.first;
             pl.gemm_alt = alt;
         }
-        for (int L : Ls)
+        for (int L : Ls) {
+            const size_t before = pl.stages.size();
             add_stage(7, L, [&](int i, const BProb &b) {
                 const long long nb = ((long long)b.m * b.n + 255) / 256;
                 for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
             });
+            if (pl.stages.size() > before)
+                for (const BProb &b : pl.probs)
+                    if (b.L == L && b.S) {
+                        pl.stages.back().nw = std::max(pl.stages.back().nw, diag_words_needed(b.S, b.w));
+                        pl.stages.back().chunked = pl.stages.back().chunked || b.nplanes > b.S;
+                    }
+        }
     } else {
         for (int L : Ls)
             add_stage(8, L, [&](int i, const BProb &b) {
@@ -552,7 +567,24 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 4: MDG_BY_L(b_slice_a_kernel); break;
     case 5: MDG_BY_L(b_slice_b_kernel); break;
     case 6: hipLaunchKernelGGL(b_diag_i8_kernel, grid, blk, 0, s, P, T); break;
-    case 7: MDG_BY_L(b_reconstruct_kernel); break;
+    case 7: {
+        auto rec = [&](auto lc) -> int {
+            constexpr int L = decltype(lc)::value;
+            return with_words<L>(st.nw, [&](auto nw) {
+                constexpr int NW = decltype(nw)::value;
+                if (st.chunked)
+                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NW, true>), grid, blk, 0, s, P, T);
+                else
+                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NW, false>), grid, blk, 0, s, P, T);
+            });
+        };
+        int rc = (st.L == 2)   ? rec(std::integral_constant<int, 2>())
+                 : (st.L == 4) ? rec(std::integral_constant<int, 4>())
+                 : (st.L == 8) ? rec(std::integral_constant<int, 8>())
+                               : rec(std::integral_constant<int, 16>());
+        if (rc) return rc;
+        break;
+    }
     default: MDG_BY_L(b_vector_kernel); break;
     }
 #undef MDG_BY_L

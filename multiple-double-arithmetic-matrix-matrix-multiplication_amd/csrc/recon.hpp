@@ -1,0 +1,185 @@
+// recon.hpp -- exact reconstruction of a slice GEMM entry from its diagonal
+// sums.
+//
+// The entry is C = sum_g D_g 2^(sc0 + w (S-1-g)), g = 0 (most significant)
+// .. S-1, with exact integer diagonal sums D_g (INT64).  The reference folds
+// such parts one at a time with inc_d (C++/TCcuda.cu:876-906); each fold is a
+// full renormalisation.  Here the whole sum is first formed exactly as an
+// integer in ws-bit word columns (ws = w * gs, gs = max(1, floor(26 / w))
+// diagonals per word: three INT8 diagonals per 24-bit word), written
+// straight into a register array -- word i holds the group of diagonals
+// nt-1-i, so no run-time register indexing and no carries until one final
+// pass -- and then folded into L limbs from its leading (2 ws)-bit piece:
+// the first L + 1 pieces through fast_renorm, the rest through inc_d.  The
+// only rounding is that final fold.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "common.hpp"
+#include "md_arith.hpp"
+
+namespace mdg {
+
+// diagonals per word: w = 8 -> 3 (24-bit words), w <= 13 -> 2, else 1; the
+// word width ws = w * gs is then >= 14 bits and <= 26
+__host__ __device__ inline int diag_group(int w) { return (26 / w) > 3 ? 3 : ((26 / w) > 1 ? 26 / w : 1); }
+
+// words that hold sum_g D_g 2^(w (S-1-g)) for |D_g| < 2^63
+inline int diag_words_needed(int S, int w)
+{
+    const int gs = diag_group(w), ws = w * gs;
+    const int nt = (S + gs - 1) / gs;
+    return nt + (63 + ws - 1) / ws + 1;
+}
+
+constexpr int RECON_BATCH = 16;   // words whose diagonal loads issue together
+
+// D(g) returns the INT64 diagonal sum g (0 <= g < S; it is called with every
+// index of the word layout, clamped into range, so that all loads issue as
+// straight-line code); GS = diag_group(w); y receives the L limbs
+template <int L, int NW, int GS, typename DiagFn>
+__device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, double (&y)[L])
+{
+    const int ws = w * GS;
+    const int nt = (S + GS - 1) / GS, pad = nt * GS - S;
+    long long z[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) z[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const int t = nt - 1 - i;
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            const int g = t * GS + u;
+            // unconditional load (clamped index) and a mask, not a branch:
+            // a branch around each load makes every load wait at its join
+            const long long on = -(long long)(t >= 0 && g < S);
+            const long long dg = D(min(max(g, 0), S - 1)) & on;
+            // D_g 2^sh, sh = w (GS-1-u): the bits below ws into word i, the
+            // rest (arithmetic shift) into word i+1 -- no overflow
+            const int sh = w * (GS - 1 - u);
+            z[i] += (dg & ((1ll << (ws - sh)) - 1)) << sh;
+            if (i + 1 < NW) z[i + 1] += dg >> (ws - sh);
+        }
+        // the loads of RB words issue together; the barrier keeps the
+        // scheduler from hoisting all of them (registers, then occupancy)
+        if (i % RECON_BATCH == RECON_BATCH - 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    // sum z_i 2^(ws i) = Z 2^(w pad), Z = sum_g D_g 2^(w (S-1-g)); carry into
+    // ws-bit words, two's complement with the final carry as the sign
+    const long long mask = (1ll << ws) - 1;
+    long long carry = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const long long v = z[i] + carry;
+        z[i] = v & mask;
+        carry = v >> ws;
+    }
+    const bool neg = carry < 0;
+    if (neg) {
+        long long cy = 1;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const long long v = (mask - z[i]) + cy;
+            z[i] = v & mask;
+            cy = v >> ws;
+        }
+    }
+    // pieces of two words (2 ws <= 52 bits, exact), the leading one first:
+    // K pieces carry at least 1 + 2 ws (K - 1) >= 53 L + 20 bits; the first
+    // L + 1 go through fast_renorm, the others one at a time through inc_d
+    constexpr int NP = (NW + 1) / 2;
+    constexpr int WSMIN = GS == 3 ? 24 : (GS == 2 ? 16 : 14);
+    constexpr int KMAX = (53 * L + 19) / (2 * WSMIN) + 2;
+    const int K = (53 * L + 19) / (2 * ws) + 2;
+    double d[NP];
+    int tp = -1;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const long long hi = (2 * p + 1 < NW) ? z[2 * p + 1] : 0;
+        d[p] = fma((double)hi, ldexp(1.0, ws), (double)z[2 * p]);
+        tp = (d[p] != 0.0) ? p : tp;
+    }
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    if (tp < 0) return;
+    const int sc = sc0 - w * pad;
+    auto piece = [&](int s) {
+        double v = 0.0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) v = (p == tp - s) ? d[p] : v;
+        return ldexp(v, 2 * ws * (tp - s) + sc);
+    };
+    {
+        double f[L + 1];
+#pragma unroll
+        for (int s = 0; s <= L; ++s) f[s] = (s < K) ? piece(s) : 0.0;
+        fast_renorm<L>(f, y);
+    }
+#pragma unroll
+    for (int s = L + 1; s < KMAX; ++s) {
+        if (s >= K) break;
+        const double v = piece(s);
+        if (v != 0.0) md_inc_d<L>(y, v);
+    }
+    if (neg) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) y[l] = -y[l];
+    }
+}
+
+// smallest word-count instantiation with at least `need` words; fn is
+// called with std::integral_constant<int, NW>
+template <int L, typename Fn>
+inline int with_words(int need, Fn fn)
+{
+    auto pick = [&](auto nw) -> bool {
+        if (decltype(nw)::value < need) return false;
+        fn(nw);
+        return true;
+    };
+    using std::integral_constant;
+    if constexpr (L == 2) {
+        if (pick(integral_constant<int, 16>()) || pick(integral_constant<int, 24>()) ||
+            pick(integral_constant<int, 32>()))
+            return MDG_SUCCESS;
+    } else if constexpr (L == 4) {
+        if (pick(integral_constant<int, 16>()) || pick(integral_constant<int, 24>()) ||
+            pick(integral_constant<int, 32>()) || pick(integral_constant<int, 48>()))
+            return MDG_SUCCESS;
+    } else if constexpr (L == 8) {
+        if (pick(integral_constant<int, 16>()) || pick(integral_constant<int, 24>()) ||
+            pick(integral_constant<int, 32>()) || pick(integral_constant<int, 48>()) ||
+            pick(integral_constant<int, 64>()))
+            return MDG_SUCCESS;
+    } else {
+        if (pick(integral_constant<int, 32>()) || pick(integral_constant<int, 48>()) ||
+            pick(integral_constant<int, 64>()) || pick(integral_constant<int, 96>()))
+            return MDG_SUCCESS;
+    }
+    set_error("no reconstruction kernel for %d diagonal-sum words at L=%d", need, L);
+    return MDG_ERROR_RANGE;
+}
+
+template <int L, typename Fn>
+inline int with_diag_words(int S, int w, Fn fn)
+{
+    return with_words<L>(diag_words_needed(S, w), fn);
+}
+
+// fn called with std::integral_constant<int, diag_group(w)>
+template <typename Fn>
+inline void with_group(int w, Fn fn)
+{
+    const int gs = diag_group(w);
+    if (gs >= 3)
+        fn(std::integral_constant<int, 3>());
+    else if (gs == 2)
+        fn(std::integral_constant<int, 2>());
+    else
+        fn(std::integral_constant<int, 1>());
+}
+
+}  // namespace mdg

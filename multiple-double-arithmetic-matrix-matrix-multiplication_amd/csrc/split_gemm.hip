@@ -40,8 +40,10 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "md_arith.hpp"
+#include "recon.hpp"
 #include "slices.hpp"
 #include "split_common.hpp"
 
@@ -266,6 +268,19 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
 // diagonal in INT64.  Plane p, in ascending diagonal order, is chunk c of
 // diagonal g with nchunks(g) = ceil((g+1) kp / kmax).
 __host__ __device__ inline int i8_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
+
+// plane index of diagonal g's first chunk: sum of the chunk counts before it
+// (closed form when kmax = c * kp: sum_{j=1..g} ceil(j / c))
+__host__ __device__ inline int i8_first_plane(int g, int kp, int kmax)
+{
+    if (kmax % kp == 0) {
+        const int c = kmax / kp, q = g / c, r = g - q * c;
+        return c * q * (q + 1) / 2 + r * (q + 1);
+    }
+    int first = 0;
+    for (int h = 0; h < g; ++h) first += i8_chunks(h, kp, kmax);
+    return first;
+}
 
 // deepest diagonal first: z-th plane of that order -> (g, c, first plane of g)
 __device__ inline void i8_plane_of(int z, int S, int kp, int kmax, int &g, int &c)
@@ -615,27 +630,9 @@ __global__ __launch_bounds__(512) void diag_f64_kernel(const TA *__restrict__ Aa
 }
 
 // ------------------------------------------------------------ reconstruct --
-// y += v * 2^sc for an exact integer v: one inc_d when v fits a double's
-// 53 bits, else two (a 32-bit-aligned high part and the low remainder)
-template <int L>
-__device__ __forceinline__ void inc_int64(double *y, long long v, int sc)
-{
-    if (v == 0) return;
-    if (v > -(1ll << 53) && v < (1ll << 53)) {
-        md_inc_d<L>(y, ldexp((double)v, sc));
-    } else {
-        const long long hi = v & ~0xFFFFFFFFll;
-        const long long lo = v - hi;   // in [0, 2^32)
-        md_inc_d<L>(y, ldexp((double)hi, sc));
-        if (lo != 0) md_inc_d<L>(y, ldexp((double)lo, sc));
-    }
-}
-
-
-// C[r][c] = fold over g of inc_d(C, D_g[r][c] * 2^(E_r + F_c + 4 - w(g+2))),
-// starting from zero, g = 0 (most significant) first.  An INT64 diagonal
-// sum is split into two exact doubles (high 32-bit-aligned part, low part).
-template <int L, typename PT>
+// C[r][c] = sum_g D_g[r][c] 2^(E_r + F_c + 4 - w(g+2)), g = 0 most
+// significant, formed exactly and folded into L limbs once (recon.hpp).
+template <int L, int NW, int GS, typename PT>
 __global__ __launch_bounds__(256) void reconstruct_kernel(int m, int n, int S, int w, const PT *__restrict__ D,
                                                           const int *__restrict__ E, const int *__restrict__ F,
                                                           double *__restrict__ C, long long ldc)
@@ -649,18 +646,17 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(int m, int n, int S, i
     const int er = E[r], fc = F[c];
     if (er != ZERO_SCALE && fc != ZERO_SCALE) {
         const long long mn = (long long)m * n;
-        for (int g = 0; g < S; ++g) {
-            const int sc = er + fc + 4 - w * (g + 2);
-            inc_int64<L>(y, (long long)D[(long long)g * mn + idx], sc);
-        }
+        diag_fold<L, NW, GS>(S, w, er + fc + 4 - w * (S + 1),
+                             [&](int g) { return (long long)D[(long long)g * mn + idx]; }, y);
     }
     double *o = C + (r * ldc + c) * L;
 #pragma unroll
     for (int l = 0; l < L; ++l) o[l] = y[l];
 }
 
-// INT8 planes: the chunks of one diagonal are summed exactly in INT64 first
-template <int L>
+// INT8 planes (w = 8): with K-chunks, the planes of one diagonal are summed
+// in INT64 first
+template <int L, int NW, bool CHUNKED>
 __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S, int kp, int kmax, int w,
                                                              const int *__restrict__ D, const int *__restrict__ E,
                                                              const int *__restrict__ F, double *__restrict__ C,
@@ -675,15 +671,19 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
     const int er = E[r], fc = F[c];
     if (er != ZERO_SCALE && fc != ZERO_SCALE) {
         const long long mn = (long long)m * n;
-        int first = 0;
-        for (int g = 0; g < S; ++g) {
-            const int nc = i8_chunks(g, kp, kmax);
-            long long v = 0;
-            for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
-            first += nc;
-            const int sc = er + fc + 4 - w * (g + 2);
-            inc_int64<L>(y, v, sc);
-        }
+        diag_fold<L, NW, 3>(S, w, er + fc + 4 - w * (S + 1),
+                            [&](int g) {
+                                if constexpr (!CHUNKED) {
+                                    return (long long)D[(long long)g * mn + idx];
+                                } else {
+                                    const int first = i8_first_plane(g, kp, kmax);
+                                    const int nc = i8_chunks(g, kp, kmax);
+                                    long long v = 0;
+                                    for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
+                                    return v;
+                                }
+                            },
+                            y);
     }
     double *o = C + (r * ldc + c) * L;
 #pragma unroll
@@ -984,8 +984,16 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         }
         lib_engine_used = lib_engine;
         tm.mark(3);
-        hipLaunchKernelGGL((reconstruct_i8_kernel<L>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, kp,
-                           kmax, w, (const int *)D, E, F, C, ldc);
+        rc = with_diag_words<L>(S, w, [&](auto nw) {
+            constexpr int NW = decltype(nw)::value;
+            if (nplanes > S)
+                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, true>), dim3(blocks_for<int>(mn, 256)), dim3(256),
+                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+            else
+                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, false>), dim3(blocks_for<int>(mn, 256)), dim3(256),
+                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+        });
+        if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
         p.wide = true;
         p.flush_blocks = 16;
@@ -1003,8 +1011,14 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         hipLaunchKernelGGL(diag_bf16_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, m, n, tiles_m, tiles_n, D);
         MDG_LAUNCH_CHECK("diag_bf16_kernel");
         tm.mark(3);
-        hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
-                           S, w, (const long long *)D, E, F, C, ldc);
+        rc = with_diag_words<L>(S, w, [&](auto nw) {
+            with_group(w, [&](auto gs) {
+                hipLaunchKernelGGL((reconstruct_kernel<L, decltype(nw)::value, decltype(gs)::value, long long>),
+                                   dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, w,
+                                   (const long long *)D, E, F, C, ldc);
+            });
+        });
+        if (rc) return rc;
     } else {
         p.wide = true;
         int32_t *Aall = (int32_t *)scratch(s, (size_t)mp * S * kp * 4, 2);
@@ -1023,8 +1037,14 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                            (long long)n);
         MDG_LAUNCH_CHECK("diag_f64_kernel");
         tm.mark(3);
-        hipLaunchKernelGGL((reconstruct_kernel<L, long long>), dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n,
-                           S, w, (const long long *)D, E, F, C, ldc);
+        rc = with_diag_words<L>(S, w, [&](auto nw) {
+            with_group(w, [&](auto gs) {
+                hipLaunchKernelGGL((reconstruct_kernel<L, decltype(nw)::value, decltype(gs)::value, long long>),
+                                   dim3(blocks_for<int>(mn, 256)), dim3(256), 0, s, m, n, S, w,
+                                   (const long long *)D, E, F, C, ldc);
+            });
+        });
+        if (rc) return rc;
     }
     MDG_LAUNCH_CHECK("split gemm");
     tm.mark(4);
