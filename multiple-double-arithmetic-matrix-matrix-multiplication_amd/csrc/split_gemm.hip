@@ -13,9 +13,10 @@
 //                       = A_all[:, 0:(g+1)kp] * Bt_all[:, (S-1-g)kp : S kp]^T,
 //                   one plain TN GEMM of depth (g+1)*kp per diagonal; all
 //                   diagonals in one launch (blockIdx.z), deepest first;
-//   4. reconstruct: C[r][c] = fold_g inc_d(C, D_g[r][c] * 2^(E_r+F_c+4-w(g+2)))
-//                   -- the TwoSum renormalisation cascade the reference's
-//                   squeeze uses (C++/TCcuda.cu:876-906).
+//   4. reconstruct: C[r][c] = sum_g D_g[r][c] 2^(E_r+F_c+4-w(g+2)), formed
+//                   exactly as an integer and folded into L limbs once with
+//                   the TwoSum renormalisation the reference's squeeze uses
+//                   (C++/TCcuda.cu:876-906; recon.hpp).
 // This replaces the reference's split -> block-Toeplitz WMMA GEMM -> squeeze
 // chain (C++/floatybits.cu:159-342, C++/TCcuda.cu:79-95, C++/TCtest.cu:21-46,
 // C++/TCcuda.cu:876-958): the slice pairs with t+u <= S-1 are the non-zero
