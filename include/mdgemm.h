@@ -210,6 +210,44 @@ int mdg_tc_gemm(int L, int m, int n, int k, const double *A, const double *B,
                 double *C, int split_kind, void *stream);
 
 /* ----------------------------------------------------------------------
+ * Elementwise p-double arithmetic (the reference's scalar library, off the
+ * GEMM path): C++/double_double_functions.cu, C++/quad_double_functions.cu,
+ * C++/octo_double_functions.cu, C++/hexa_double_functions.cu.
+ * -------------------------------------------------------------------- */
+
+typedef enum mdg_op {
+    MDG_OP_ADD = 0,       /* c = a + b   Xdf_add (quad :216-242)           */
+    MDG_OP_SUB = 1,       /* c = a - b   Xdf_sub (double :61-73, quad :299-309) */
+    MDG_OP_MUL = 2,       /* c = a * b   Xdf_mul (quad :321-381)           */
+    MDG_OP_DIV = 3,       /* c = a / b   Xdf_div (double :230-248, quad :485-520) */
+    MDG_OP_SQR = 4,       /* c = a * a   Xdf_sqr (double :208-216, quad :383-440) */
+    MDG_OP_SQRT = 5,      /* c = sqrt(a) Xdf_sqrt (double :250-279, quad :524-537);
+                             a negative leading limb gives the reference's -1
+                             root at double-double and whatever its Heron
+                             step then makes of it above */
+    MDG_OP_MUL_D = 6,     /* c = a * d   ddf_mul_d_dd (double :218-228),
+                             Xdf_mul_X_d (quad :442-468)                  */
+    MDG_OP_INC_D = 7,     /* c = a + d   Xdf_inc_d (double :99-106, quad :269-275) */
+    MDG_OP_MUL_PWR2 = 8,  /* c = a * d, d a power of two: limb-wise
+                             (quad :311-319)                             */
+    MDG_OP_ABS = 9,       /* c = |a|     Xdf_abs (quad :194-212); at L = 16
+                             the reference copies limb 5 into limb 9
+                             (hexa :1479,1489), this returns |a| limb-wise */
+    MDG_OP_NEG = 10       /* c = -a      Xdf_minus (quad :290-297)         */
+} mdg_op;
+
+/* c[i] = op(a[i], b[i]) for n entries of L-limb numbers in the interleaved
+   layout (entry i at [i * L + limb]).  b: n L-limb entries for ADD, SUB, MUL
+   and DIV; n doubles (b[i]) for MUL_D, INC_D and MUL_PWR2; ignored (may be
+   NULL) for the unary ops.  c may alias a or b entry for entry.  twoprod
+   selects the exact product (mdg_twoprod): MDG_TWOPROD_DEKKER gives the
+   reference's host bits; the two agree wherever no partial product
+   underflows.  Division by zero and roots of zero (L >= 4) give Inf/NaN as
+   the reference's do; no input is rejected for its value. */
+int mdg_elementwise(int op, int L, long long n, const double *a, const double *b, double *c, int twoprod,
+                    void *stream);
+
+/* ----------------------------------------------------------------------
  * Utilities.
  * -------------------------------------------------------------------- */
 

@@ -48,6 +48,25 @@ std::vector<double> matmulhdf(std::vector<double> A, std::vector<double> B, int 
 // C++/TCcuda.h:103 -- fold pp parts into p-doubles, kernel time into t_CUDA (ms)
 std::vector<double> pllsqueeze_old(std::vector<double> x, int p, int pp, float &t_CUDA);
 
+// C++/TCcuda.h:86-96 -- the same fold, y = inc_d(...inc_d(0, x_0)..., x_{pp-1}),
+// under the reference's per-(p, pp) names: pllsqueeze_2_{8,12,16} (its GPU
+// versions) and pllntsqueeze_2_{8,12,16} (its host loops; here on the GPU
+// too, the same bits), dispatched by pllsqueeze / pllntsqueeze (C++/TCcuda.cu
+// :810-875).  Differences: every entry is folded (the reference's kernel
+// skips the last n % 64 entries when n > 64, C++/TCcuda.cu:454-466), and
+// pllsqueeze(x, p, pp) serves every p in {2, 4, 8, 16} and pp >= 1 where the
+// reference returns a double-double for pp == 12 whatever p is and falls off
+// the end of the function for the other pairs; here pp == 12 with p != 2
+// and any other unsupported p throw.
+std::vector<double> pllsqueeze_2_8(std::vector<double> x);
+std::vector<double> pllsqueeze_2_12(std::vector<double> x);
+std::vector<double> pllsqueeze_2_16(std::vector<double> x);
+std::vector<double> pllntsqueeze_2_8(std::vector<double> x);
+std::vector<double> pllntsqueeze_2_12(std::vector<double> x);
+std::vector<double> pllntsqueeze_2_16(std::vector<double> x);
+std::vector<double> pllsqueeze(std::vector<double> x, int p, int pp);
+std::vector<double> pllntsqueeze(std::vector<double> x, int p, int pp);
+
 // C++/floatybits.h:72,87,97 -- the bit splitters
 std::vector<double> split4pd(std::vector<double> x, int p);
 std::vector<double> split8pd(std::vector<double> x, int p);

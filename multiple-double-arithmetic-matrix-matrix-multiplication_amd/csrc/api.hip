@@ -19,6 +19,9 @@ int conv_gemm(int m, int n, int k, int pp, const double *Aq, const double *Bq, d
 int conv_squeeze(int L, int m, int n, int k, int pp, const double *Aq, const double *Bq, double *C, hipStream_t s);
 int ref_split(int kind, const double *x, long long count, int p, double *out, hipStream_t s);
 int squeeze(int L, long long nent, int pp, const double *x, double *y, hipStream_t s);
+int elementwise(int op, int L, long long n, const double *a, const double *b, double *c, int tp, hipStream_t s);
+bool elementwise_binary(int op);
+bool elementwise_takes_double(int op);
 int bigB2(const double *Bq, int n, int pp, double *out, hipStream_t s);
 int fill_random(int kind, int L, long long count, unsigned long long seed, int emin, int emax, double *out,
                 hipStream_t s);
@@ -311,6 +314,23 @@ int mdg_squeeze(int L, long long nent, int pp, const double *x, double *y, void 
     int rc = ensure_device();
     if (rc) return rc;
     return squeeze(L, nent, pp, x, y, as_stream(stream));
+}
+
+int mdg_elementwise(int op, int L, long long n, const double *a, const double *b, double *c, int twoprod,
+                    void *stream)
+{
+    MDG_REQUIRE(op >= MDG_OP_ADD && op <= MDG_OP_NEG, MDG_ERROR_INVALID_VALUE, "unsupported elementwise op %d", op);
+    MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d", L);
+    MDG_REQUIRE(n >= 0, MDG_ERROR_INVALID_VALUE, "negative entry count %lld", n);
+    MDG_REQUIRE(twoprod == MDG_TWOPROD_FMA || twoprod == MDG_TWOPROD_DEKKER, MDG_ERROR_INVALID_VALUE,
+                "bad twoprod %d", twoprod);
+    const bool needs_b = elementwise_binary(op) || elementwise_takes_double(op);
+    MDG_REQUIRE(n == 0 || (a && c && (b || !needs_b)), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    MDG_REQUIRE(n <= (1ll << 40), MDG_ERROR_INVALID_VALUE, "entry count %lld too large", n);
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (n == 0) return MDG_SUCCESS;
+    return elementwise(op, L, n, a, b, c, twoprod, as_stream(stream));
 }
 
 int mdg_tc_gemm(int L, int m, int n, int k, const double *A, const double *B, double *C, int split_kind,

@@ -189,6 +189,39 @@ std::vector<double> pllsqueeze_old(std::vector<double> x, int p, int pp, float &
     return dy.host();
 }
 
+namespace {
+
+std::vector<double> squeeze_parts(const std::vector<double> &x, int p, int pp, const char *what)
+{
+    need(p == 2 || p == 4 || p == 8 || p == 16, "p must be 2, 4, 8 or 16");
+    need(pp >= 1 && x.size() % (size_t)pp == 0, "x must hold a multiple of pp doubles");
+    need(pp != 12 || p == 2, "pp == 12 (mixsplit2 parts) is a double-double split");
+    const long long nent = (long long)(x.size() / pp);
+    DevBuf dx(x), dy((size_t)nent * p);
+    throw_if(mdg_squeeze(p, nent, pp, dx.p, dy.p, nullptr), what);
+    return dy.host();
+}
+
+}  // namespace
+
+std::vector<double> pllsqueeze_2_8(std::vector<double> x) { return squeeze_parts(x, 2, 8, "pllsqueeze_2_8"); }
+std::vector<double> pllsqueeze_2_12(std::vector<double> x) { return squeeze_parts(x, 2, 12, "pllsqueeze_2_12"); }
+std::vector<double> pllsqueeze_2_16(std::vector<double> x) { return squeeze_parts(x, 2, 16, "pllsqueeze_2_16"); }
+std::vector<double> pllntsqueeze_2_8(std::vector<double> x) { return squeeze_parts(x, 2, 8, "pllntsqueeze_2_8"); }
+std::vector<double> pllntsqueeze_2_12(std::vector<double> x)
+{
+    return squeeze_parts(x, 2, 12, "pllntsqueeze_2_12");
+}
+std::vector<double> pllntsqueeze_2_16(std::vector<double> x)
+{
+    return squeeze_parts(x, 2, 16, "pllntsqueeze_2_16");
+}
+std::vector<double> pllsqueeze(std::vector<double> x, int p, int pp) { return squeeze_parts(x, p, pp, "pllsqueeze"); }
+std::vector<double> pllntsqueeze(std::vector<double> x, int p, int pp)
+{
+    return squeeze_parts(x, p, pp, "pllntsqueeze");
+}
+
 std::vector<double> split4pd(std::vector<double> x, int p)
 {
     DevBuf dx(x), out(4 * x.size());

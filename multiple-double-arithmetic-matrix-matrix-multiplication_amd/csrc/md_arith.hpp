@@ -281,4 +281,182 @@ __device__ __forceinline__ void md_inc_d(double *a, double y)
     }
 }
 
+// ------------------------------------------- operations off the GEMM path --
+// sub, sqr, mul by a double, div, sqrt with the reference's operation order:
+//   dd:  ddf_sub :61-73, ddf_two_sqr :186-195, ddf_sqr :208-216,
+//        ddf_mul_d_dd :218-228, ddf_div :230-248, ddf_sqrt :250-279
+//   qd/od/hd: X_sub (copy, minus, inc) quad :299-309, octo :684-704,
+//        hexa :1957-2005; X_mul_X_d (baileyMul_fast<L,1,L>) quad :442-468,
+//        octo :1305-1365, hexa :5676-5858; X_div quad :485-520, octo
+//        :1390-1505, hexa :5903-6296; X_sqrt quad :524-537, octo :1507-1536,
+//        hexa :6298-6361.  The qd/od/hd sqr bodies are the mul bodies with
+//        b = a (quad :383-440 against :321-381), so md_sqr is md_mul(a, a).
+
+__device__ __forceinline__ double two_diff(double a, double b, double &e)
+{
+    double s = a - b;
+    double bb = s - a;
+    e = (a - (s - bb)) - (b + bb);
+    return s;
+}
+
+__device__ __forceinline__ void dd_sub(const double *a, const double *b, double *c)
+{
+    double s1, s2, t1, t2;
+    s1 = two_diff(a[0], b[0], s2);
+    t1 = two_diff(a[1], b[1], t2);
+    s2 += t1;
+    s1 = quick_two_sum(s1, s2, s2);
+    s2 += t2;
+    double c1;
+    double c0 = quick_two_sum(s1, s2, c1);
+    c[0] = c0;
+    c[1] = c1;
+}
+
+template <int TP>
+__device__ __forceinline__ void dd_sqr(const double *a, double *c)
+{
+    double p2;
+    const double p1 = a[0] * a[0];
+    if constexpr (TP == TP_FMA) {
+        p2 = __builtin_fma(a[0], a[0], -p1);
+    } else {
+        double hi, lo;
+        veltkamp(a[0], hi, lo);
+        p2 = ((hi * hi - p1) + 2.0 * hi * lo) + lo * lo;
+    }
+    p2 += 2.0 * a[0] * a[1];
+    p2 += a[1] * a[1];
+    double c1;
+    double c0 = quick_two_sum(p1, p2, c1);
+    c[0] = c0;
+    c[1] = c1;
+}
+
+// c = a * d for a double d
+template <int L, int TP>
+__device__ __forceinline__ void md_mul_d(const double *a, double d, double *c)
+{
+    if constexpr (L == 2) {
+        double p2;
+        const double p1 = two_prod<TP>(d, a[0], p2);
+        p2 += (a[1] * d);
+        double c1;
+        double c0 = quick_two_sum(p1, p2, c1);
+        c[0] = c0;
+        c[1] = c1;
+    } else {
+        double f[L + 1], e;
+        f[L] = 0.0;
+        f[L - 1] = two_prod<TP>(a[L - 1], d, e);
+        f[L] += e;
+#pragma unroll
+        for (int i = L - 2; i >= 0; --i) {
+            f[i] = two_prod<TP>(a[i], d, e);
+#pragma unroll
+            for (int j = i + 1; j < L; ++j) f[j] = two_sum(f[j], e, e);
+            f[L] += e;
+        }
+        fast_renorm<L>(f, c);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void md_sub(const double *a, const double *b, double *c)
+{
+    if constexpr (L == 2) {
+        dd_sub(a, b, c);
+    } else {
+        double t[L];
+#pragma unroll
+        for (int i = 0; i < L; ++i) t[i] = -b[i];
+        md_add<L>(t, a, c);
+    }
+}
+
+template <int L, int TP>
+__device__ __forceinline__ void md_sqr(const double *a, double *c)
+{
+    if constexpr (L == 2)
+        dd_sqr<TP>(a, c);
+    else
+        md_mul<L, TP>(a, a, c);
+}
+
+// quotient digits q_i = r_0 / b_0, r -= q_i b: three for dd (the last two
+// folded in by quick_two_sum and inc_d), L + 1 through fast_renorm otherwise
+template <int L, int TP>
+__device__ __forceinline__ void md_div(const double *a, const double *b, double *c)
+{
+    double acc[L], r[L], t[L];
+    if constexpr (L == 2) {
+        const double q1 = a[0] / b[0];
+        md_mul_d<2, TP>(b, q1, acc);
+        dd_sub(a, acc, r);
+        const double q2 = r[0] / b[0];
+        md_mul_d<2, TP>(b, q2, acc);
+        dd_sub(r, acc, t);
+        const double q3 = t[0] / b[0];
+        double c1;
+        double c0 = quick_two_sum(q1, q2, c1);
+        c[0] = c0;
+        c[1] = c1;
+        dd_inc_d(c, q3);
+    } else {
+        double q[L + 1];
+        q[0] = a[0] / b[0];
+        md_mul_d<L, TP>(b, q[0], acc);
+        md_sub<L>(a, acc, r);
+#pragma unroll
+        for (int i = 1; i < L; ++i) {
+            q[i] = r[0] / b[0];
+            md_mul_d<L, TP>(b, q[i], acc);
+            md_sub<L>(r, acc, t);
+#pragma unroll
+            for (int j = 0; j < L; ++j) r[j] = t[j];
+        }
+        q[L] = r[0] / b[0];
+        fast_renorm<L>(q, c);
+    }
+}
+
+// dd: Karp's step from 1/sqrt(a_0), 0 for 0 and -1 for a negative leading
+// limb (the reference's convention); L >= 4: the root s of the leading L/2
+// limbs, then one Heron step (s^2 + a) / s * 0.5 at full length
+template <int L, int TP>
+__device__ __forceinline__ void md_sqrt(const double *a, double *b)
+{
+    if constexpr (L == 2) {
+        if (a[0] == 0.0 && a[1] == 0.0) {
+            b[0] = 0.0;
+            b[1] = 0.0;
+        } else if (a[0] < 0.0) {
+            b[0] = -1.0;
+            b[1] = 0.0;
+        } else {
+            const double x = 1.0 / sqrt(a[0]);
+            const double ax = a[0] * x;
+            const double s[2] = {ax, 0.0};
+            double sq[2], y[2];
+            dd_sqr<TP>(s, sq);
+            dd_sub(a, sq, y);
+            double b1;
+            const double b0 = two_sum(ax, y[0] * x * 0.5, b1);
+            b[0] = b0;
+            b[1] = b1;
+        }
+    } else {
+        double s[L], z[L], t[L];
+        md_sqrt<L / 2, TP>(a, s);
+#pragma unroll
+        for (int i = L / 2; i < L; ++i) s[i] = 0.0;
+        md_mul<L, TP>(s, s, z);
+        md_add<L>(z, a, t);
+        md_div<L, TP>(t, s, z);
+#pragma unroll
+        for (int i = 0; i < L; ++i) b[i] = z[i] * 0.5;
+    }
+}
+
 }  // namespace mdg

@@ -15,6 +15,11 @@ METHOD_VECTOR, METHOD_F64_SPLIT, METHOD_INT8_OZAKI, METHOD_BF16_OZAKI = 0, 1, 2,
 METHODS = {"vector": 0, "f64": 1, "int8": 2, "bf16": 3, "int8_residue": 4, "f64_residue": 5}
 TWOPROD_FMA, TWOPROD_DEKKER = 0, 1
 GEN_DD, GEN_QD, GEN_REF = 0, 1, 2
+# elementwise operations (mdg_op)
+OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqr": 4, "sqrt": 5, "mul_d": 6, "inc_d": 7,
+       "mul_pwr2": 8, "abs": 9, "neg": 10}
+BINARY_OPS = ("add", "sub", "mul", "div")
+DOUBLE_OPS = ("mul_d", "inc_d", "mul_pwr2")
 
 STATUS = {0: "MDG_SUCCESS", 1: "MDG_ERROR_INVALID_VALUE", 2: "MDG_ERROR_NOT_SUPPORTED",
           3: "MDG_ERROR_HIP", 4: "MDG_ERROR_OUT_OF_MEMORY", 5: "MDG_ERROR_NON_FINITE",
@@ -69,6 +74,7 @@ _SIGS = {
     "mdg_squeeze": ([_i, _ll, _i, _vp, _vp, _vp], _i),
     "mdg_tc_gemm": ([_i, _i, _i, _i, _vp, _vp, _vp, _i, _vp], _i),
     "mdg_fill_random": ([_i, _i, _ll, _ull, _i, _i, _vp, _vp], _i),
+    "mdg_elementwise": ([_i, _i, _ll, _vp, _vp, _vp, _i, _vp], _i),
 }
 
 
@@ -228,4 +234,16 @@ def squeeze(L, pp, x, stream=None):
 def tc_gemm(L, m, n, k, A, B, split_kind=4, stream=None):
     out = empty(m * n * L)
     _check(lib().mdg_tc_gemm(L, m, n, k, _ptr(A), _ptr(B), _ptr(out), split_kind, _stream(stream)))
+    return out
+
+
+def elementwise(op, L, a, b=None, twoprod=TWOPROD_FMA, out=None, stream=None):
+    """c = op(a, b) entry by entry on L-limb interleaved arrays (mdg_elementwise).
+    op: a name of OPS or its code; b: L-limb entries for add/sub/mul/div, one
+    double per entry for mul_d/inc_d/mul_pwr2, unused otherwise."""
+    code = OPS[op] if isinstance(op, str) else int(op)
+    n = a.numel() // L
+    if out is None:
+        out = empty(n * L)
+    _check(lib().mdg_elementwise(code, L, n, _ptr(a), _ptr(b), _ptr(out), twoprod, _stream(stream)))
     return out

@@ -27,6 +27,11 @@
  *                                            octo :723-1014, hexa :2044-3863
  *                          inc_d             quad :269-275, octo :637-646,
  *                                            hexa :1871-1890
+ *                          sub, sqr, mul by a double, div, sqrt, abs, minus,
+ *                          mul_pwr2          double :47-73,186-279,
+ *                                            quad :194-212,290-537,
+ *                                            octo :669-722,1016-1536,
+ *                                            hexa :1937-2043,3865-6361
  *   L3  host GEMM          matmul{dd,qd,od,hd}f  C++/TCcuda.cu:320-372
  *       squeeze            pllsqueeze_{2,4,8,16}_pp_old  C++/TCcuda.cu:876-906
  *       limb convolution   bigB2 + WMMA matmul  C++/TCcuda.cu:79-95,
@@ -308,6 +313,241 @@ static inline void any_inc_d(int L, double *a, double y)
 }
 
 /* ------------------------------------------------------------------ */
+/* L1: the scalar operations off the GEMM path: sub, sqr, mul by a     */
+/* double, div, sqrt (plus the trivial abs, negation, power-of-two     */
+/* scaling).  The reference spells them out per L; the qd/od/hd sqr    */
+/* bodies are the mul bodies with b replaced by a, and the qd/od/hd    */
+/* inc bodies are the add bodies, so both are restated as those.       */
+/* ------------------------------------------------------------------ */
+
+/* two_diff, double_double_functions.cu:47-53: s + e == a - b exactly */
+static inline double tdiff(double a, double b, double *e)
+{
+    double s = a - b;
+    double bb = s - a;
+    *e = (a - (s - bb)) - (b + bb);
+    return s;
+}
+
+/* ddf_sub :61-73 (ddf_dec :108-118 is the same with c == a) */
+static inline void dd_sub(const double *a, const double *b, double *c)
+{
+    double s1, s2, t1, t2;
+    s1 = tdiff(a[0], b[0], &s2);
+    t1 = tdiff(a[1], b[1], &t2);
+    s2 += t1;
+    s1 = qts(s1, s2, &s2);
+    s2 += t2;
+    double c0, c1;
+    c0 = qts(s1, s2, &c1);
+    c[0] = c0;
+    c[1] = c1;
+}
+
+/* ddf_two_sqr :186-195 (Dekker), or through fma */
+static inline double two_sqr(double a, double *e, int mode)
+{
+    double q = a * a;
+    if (mode) {
+        *e = fma(a, a, -q);
+    } else {
+        double hi, lo;
+        vsplit(a, &hi, &lo);
+        *e = ((hi * hi - q) + 2.0 * hi * lo) + lo * lo;
+    }
+    return q;
+}
+
+/* ddf_sqr :208-216 */
+static inline void dd_sqr(const double *a, double *c, int mode)
+{
+    double p1, p2;
+    p1 = two_sqr(a[0], &p2, mode);
+    p2 += 2.0 * a[0] * a[1];
+    p2 += a[1] * a[1];
+    double c0, c1;
+    c0 = qts(p1, p2, &c1);
+    c[0] = c0;
+    c[1] = c1;
+}
+
+/* ddf_mul_d_dd :218-228, c = d * b for a double d */
+static inline void dd_mul_d(double d, const double *b, double *c, int mode)
+{
+    double p1, p2;
+    p1 = tp(d, b[0], &p2, mode);
+    p2 += (b[1] * d);
+    double c0, c1;
+    c0 = qts(p1, p2, &c1);
+    c[0] = c0;
+    c[1] = c1;
+}
+
+/* ddf_div :230-248: three quotient digits */
+static inline void dd_div(const double *a, const double *b, double *c, int mode)
+{
+    double q1, q2, q3, acc[2], r[2], t[2];
+    q1 = a[0] / b[0];
+    dd_mul_d(q1, b, acc, mode);
+    dd_sub(a, acc, r);
+    q2 = r[0] / b[0];
+    dd_mul_d(q2, b, acc, mode);
+    dd_sub(r, acc, t);
+    q3 = t[0] / b[0];
+    double c0, c1;
+    c0 = qts(q1, q2, &c1);
+    c[0] = c0;
+    c[1] = c1;
+    dd_inc_d(c, q3);
+}
+
+/* ddf_sqrt :250-279: Karp's step from 1/sqrt(a_hi); zero gives zero and a
+   negative leading limb gives -1 (the reference's convention) */
+static inline void dd_sqrt(const double *a, double *b, int mode)
+{
+    if (a[0] == 0.0 && a[1] == 0.0) {
+        b[0] = 0.0;
+        b[1] = 0.0;
+    } else if (a[0] < 0.0) {
+        b[0] = -1.0;
+        b[1] = 0.0;
+    } else {
+        const double x = 1.0 / sqrt(a[0]);
+        const double ax = a[0] * x;
+        double s[2] = {ax, 0.0}, sq[2], y[2];
+        dd_sqr(s, sq, mode);
+        dd_sub(a, sq, y);
+        double b0, b1;
+        b0 = ts(ax, y[0] * x * 0.5, &b1);
+        b[0] = b0;
+        b[1] = b1;
+    }
+}
+
+/* qdf_sub :299-309, odf_sub :684-704, hdf_sub :1957-2005:
+   c = -b, then c += a (inc, whose body is add's) */
+static inline void md_sub(int L, const double *a, const double *b, double *c)
+{
+    double t[MAXL];
+    for (int i = 0; i < L; ++i)
+        t[i] = -b[i];
+    md_add(L, t, a, c);
+}
+
+/* CAMPARY baileyMul_fast<L,1,L>: qdf_mul_qd_d :442-468, odf_mul_od_d
+   :1305-1365, hdf_mul_hd_d :5676-5858.  Limb products from the least
+   significant up, each error rippled down the lower partial sums. */
+static inline void md_mul_d(int L, const double *a, double d, double *c, int mode)
+{
+    double f[MAXL + 1], e;
+    f[L] = 0.0;
+    f[L - 1] = tp(a[L - 1], d, &e, mode);
+    f[L] += e;
+    for (int i = L - 2; i >= 0; --i) {
+        f[i] = tp(a[i], d, &e, mode);
+        for (int j = i + 1; j < L; ++j)
+            f[j] = ts(f[j], e, &e);
+        f[L] += e;
+    }
+    fast_renorm(L, f, c);
+}
+
+/* qdf_div :485-520, odf_div :1390-1505, hdf_div :5903-6296: L+1 quotient
+   digits q_i = r_0 / b_0 with r -= q_i b, then fast_renorm of the digits */
+static inline void md_div(int L, const double *a, const double *b, double *c, int mode)
+{
+    double q[MAXL + 1], acc[MAXL], r[MAXL], t[MAXL];
+    q[0] = a[0] / b[0];
+    md_mul_d(L, b, q[0], acc, mode);
+    md_sub(L, a, acc, r);
+    for (int i = 1; i < L; ++i) {
+        q[i] = r[0] / b[0];
+        md_mul_d(L, b, q[i], acc, mode);
+        md_sub(L, r, acc, t);
+        memcpy(r, t, sizeof(double) * (size_t)L);
+    }
+    q[L] = r[0] / b[0];
+    fast_renorm(L, q, c);
+}
+
+static void any_sqrt(int L, const double *a, double *b, int mode);
+
+/* qdf_sqrt :524-537, odf_sqrt :1507-1536, hdf_sqrt :6298-6361: the root of
+   the leading L/2 limbs at half precision, then one Heron step
+   b = (s^2 + a) / s * 0.5 with s that root padded with zero limbs */
+static inline void md_sqrt(int L, const double *a, double *b, int mode)
+{
+    double s[MAXL], z[MAXL], t[MAXL];
+    const int h = L / 2;
+    any_sqrt(h, a, s, mode);
+    for (int i = h; i < L; ++i)
+        s[i] = 0.0;
+    md_mul(L, s, s, z, mode);
+    md_add(L, z, a, t);
+    md_div(L, t, s, z, mode);
+    for (int i = 0; i < L; ++i)
+        b[i] = z[i] * 0.5;
+}
+
+static void any_sqrt(int L, const double *a, double *b, int mode)
+{
+    if (L == 2)
+        dd_sqrt(a, b, mode);
+    else
+        md_sqrt(L, a, b, mode);
+}
+
+/* Operation codes, equal to the product library's mdg_op values. */
+enum {
+    OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_SQR = 4, OP_SQRT = 5,
+    OP_MUL_D = 6, OP_INC_D = 7, OP_MUL_PWR2 = 8, OP_ABS = 9, OP_NEG = 10
+};
+
+/* One entry of an elementwise operation: a and c L-limb, b L-limb for the
+   binary ops, b[0] a double for the _D/_PWR2 ops, unused otherwise. */
+static int one_op(int op, int L, const double *a, const double *b, double *c, int mode)
+{
+    double t[MAXL];
+    switch (op) {
+    case OP_ADD: any_add(L, a, b, c); break;
+    case OP_SUB:
+        if (L == 2) dd_sub(a, b, c);
+        else md_sub(L, a, b, c);
+        break;
+    case OP_MUL: any_mul(L, a, b, c, mode); break;
+    case OP_DIV:
+        if (L == 2) dd_div(a, b, c, mode);
+        else md_div(L, a, b, c, mode);
+        break;
+    case OP_SQR:
+        if (L == 2) dd_sqr(a, c, mode);
+        else md_mul(L, a, a, c, mode);
+        break;
+    case OP_SQRT: any_sqrt(L, a, c, mode); break;
+    case OP_MUL_D:
+        if (L == 2) dd_mul_d(b[0], a, c, mode);
+        else md_mul_d(L, a, b[0], c, mode);
+        break;
+    case OP_INC_D:
+        memcpy(t, a, sizeof(double) * (size_t)L);
+        any_inc_d(L, t, b[0]);
+        memcpy(c, t, sizeof(double) * (size_t)L);
+        break;
+    case OP_MUL_PWR2:   /* qdf_mul_pwr2 :311-319: limb-wise, exact */
+        for (int i = 0; i < L; ++i) c[i] = a[i] * b[0];
+        break;
+    case OP_ABS:        /* qdf_abs :194-212: sign of the leading limb */
+        for (int i = 0; i < L; ++i) c[i] = a[0] < 0.0 ? -a[i] : a[i];
+        break;
+    case OP_NEG:        /* qdf_minus :290-297 */
+        for (int i = 0; i < L; ++i) c[i] = -a[i];
+        break;
+    default: return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Exported per-operation entry points (vectorised over `count`).      */
 /* ------------------------------------------------------------------ */
 
@@ -360,6 +600,19 @@ int orc_renorm(int L, long count, const double *f, double *r)
     if (L != 4 && L != 8 && L != 16) return 1;
     for (long i = 0; i < count; ++i)
         renorm_ladder(L, f + i * (L + 1), r + i * L);
+    return 0;
+}
+
+/* Elementwise operation `op` over `count` entries (operand strides: L for
+   a, c and the binary ops' b, 1 for the double operand of the _D/_PWR2
+   ops). */
+int orc_op(int op, int L, long count, const double *a, const double *b, double *c, int mode)
+{
+    if (!valid_L(L) || op < OP_ADD || op > OP_NEG) return 1;
+    const int bstride = (op == OP_MUL_D || op == OP_INC_D || op == OP_MUL_PWR2) ? 1
+                        : (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV) ? L : 0;
+    for (long i = 0; i < count; ++i)
+        if (one_op(op, L, a + i * L, b ? b + i * bstride : NULL, c + i * L, mode)) return 1;
     return 0;
 }
 

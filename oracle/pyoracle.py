@@ -32,6 +32,12 @@ GEN_DD = 0     # hi ~ U[-1,1), lo = hi*2^-53*U[-.5,.5)
 GEN_QD = 1     # lead s*2^e, e in [emin,emax], next = prev*2^-53*U[-.5,.5)
 GEN_REF = 2    # the reference's random_pd layout (positive, 52-bit limbs)
 
+# elementwise operation codes (the product library's mdg_op values)
+OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_SQR, OP_SQRT = 0, 1, 2, 3, 4, 5
+OP_MUL_D, OP_INC_D, OP_MUL_PWR2, OP_ABS, OP_NEG = 6, 7, 8, 9, 10
+BINARY_OPS = (OP_ADD, OP_SUB, OP_MUL, OP_DIV)
+DOUBLE_OPS = (OP_MUL_D, OP_INC_D, OP_MUL_PWR2)
+
 
 def build(force=False):
     """Compile libmdoracle.so (and _ref when the reference tree exists)."""
@@ -67,6 +73,8 @@ def lib():
         L.orc_fast_renorm.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp]
         L.orc_renorm.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp]
         L.orc_two_prod.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
+                             ctypes.c_int]
         L.orc_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                _dp, ctypes.c_long, _dp, ctypes.c_long, _dp, ctypes.c_long,
                                ctypes.c_int, ctypes.c_int]
@@ -102,6 +110,7 @@ def ref():
         R.ref_fast_renorm.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp]
         R.ref_renorm.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp]
         R.ref_two_prod.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp]
+        R.ref_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp]
         R.ref_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                _dp, ctypes.c_long, _dp, ctypes.c_long, _dp, ctypes.c_long]
         R.ref_squeeze.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_int, _dp, _dp]
@@ -150,6 +159,24 @@ def op_inc_d(L, a, y, use_ref=False):
     c = np.empty_like(a)
     fn = ref().ref_inc_d if use_ref else lib().orc_inc_d
     _chk(fn(L, a.size // L, _p(a), _p(y), _p(c)), "inc_d")
+    return c
+
+
+def op(code, L, a, b=None, mode=TP_DEKKER, use_ref=False):
+    """Elementwise operation `code` (OP_*) on L-limb entries a (and b: L-limb
+    entries for the binary ops, one double per entry for the _D/_PWR2 ops)."""
+    a = _c(a)
+    cnt = a.size // L
+    c = np.empty_like(a)
+    bp = None
+    if code in BINARY_OPS or code in DOUBLE_OPS:
+        b = _c(b)
+        assert b.size == (a.size if code in BINARY_OPS else cnt)
+        bp = _p(b)
+    if use_ref:
+        _chk(ref().ref_op(code, L, cnt, _p(a), bp, _p(c)), f"ref_op {code}")
+    else:
+        _chk(lib().orc_op(code, L, cnt, _p(a), bp, _p(c), mode), f"op {code}")
     return c
 
 

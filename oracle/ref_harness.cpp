@@ -60,6 +60,79 @@ static void ref_inc_d1(int L, double *a, double y)
 
 static bool ok_L(int L) { return L == 2 || L == 4 || L == 8 || L == 16; }
 
+// The reference's off-path scalar operations, one entry; op codes as in
+// md_oracle.c (0 add, 1 sub, 2 mul, 3 div, 4 sqr, 5 sqrt, 6 mul by a double,
+// 7 inc_d, 8 mul_pwr2, 9 abs, 10 minus).
+static int ref_op1(int op, int L, const double *a, const double *b, double *c)
+{
+    double t[16];
+    std::memcpy(t, a, sizeof(double) * L);
+    switch (op) {
+    case 0: ref_add1(L, a, b, c); return 0;
+    case 2: ref_mul1(L, a, b, c); return 0;
+    case 7:
+        ref_inc_d1(L, t, b[0]);
+        std::memcpy(c, t, sizeof(double) * L);
+        return 0;
+    case 10:
+        switch (L) {
+        case 2: ddf_minus(&t[0], &t[1]); break;
+        case 4: qdf_minus(P4(t)); break;
+        case 8: odf_minus(P8(t)); break;
+        case 16: hdf_minus(P16(t)); break;
+        }
+        std::memcpy(c, t, sizeof(double) * L);
+        return 0;
+    }
+    switch (L) {
+    case 2:
+        switch (op) {
+        case 1: ddf_sub(a[0], a[1], b[0], b[1], &c[0], &c[1]); return 0;
+        case 3: ddf_div(a[0], a[1], b[0], b[1], &c[0], &c[1]); return 0;
+        case 4: ddf_sqr(a[0], a[1], &c[0], &c[1]); return 0;
+        case 5: ddf_sqrt(a[0], a[1], &c[0], &c[1]); return 0;
+        case 6: ddf_mul_d_dd(b[0], a[0], a[1], &c[0], &c[1]); return 0;
+        case 8: c[0] = a[0] * b[0]; c[1] = a[1] * b[0]; return 0;
+        case 9: ddf_abs(a[0], a[1], &c[0], &c[1]); return 0;
+        }
+        break;
+    case 4:
+        switch (op) {
+        case 1: qdf_sub(V4(a), V4(b), P4(c)); return 0;
+        case 3: qdf_div(V4(a), V4(b), P4(c)); return 0;
+        case 4: qdf_sqr(V4(a), P4(c)); return 0;
+        case 5: qdf_sqrt(V4(a), P4(c)); return 0;
+        case 6: qdf_mul_qd_d(V4(a), b[0], P4(c)); return 0;
+        case 8: qdf_mul_pwr2(V4(a), b[0], P4(c)); return 0;
+        case 9: qdf_abs(V4(a), P4(c)); return 0;
+        }
+        break;
+    case 8:
+        switch (op) {
+        case 1: odf_sub(V8(a), V8(b), P8(c)); return 0;
+        case 3: odf_div(V8(a), V8(b), P8(c)); return 0;
+        case 4: odf_sqr(V8(a), P8(c)); return 0;
+        case 5: odf_sqrt(V8(a), P8(c)); return 0;
+        case 6: odf_mul_od_d(V8(a), b[0], P8(c)); return 0;
+        case 8: odf_mul_pwr2(V8(a), b[0], P8(c)); return 0;
+        case 9: odf_abs(V8(a), P8(c)); return 0;
+        }
+        break;
+    case 16:
+        switch (op) {
+        case 1: hdf_sub(V16(a), V16(b), P16(c)); return 0;
+        case 3: hdf_div(V16(a), V16(b), P16(c)); return 0;
+        case 4: hdf_sqr(V16(a), P16(c)); return 0;
+        case 5: hdf_sqrt(V16(a), P16(c)); return 0;
+        case 6: hdf_mul_hd_d(V16(a), b[0], P16(c)); return 0;
+        case 8: hdf_mul_pwr2(V16(a), b[0], P16(c)); return 0;
+        case 9: hdf_abs(V16(a), P16(c)); return 0;
+        }
+        break;
+    }
+    return 1;
+}
+
 extern "C" {
 
 int ref_mul(int L, long count, const double *a, const double *b, double *c)
@@ -85,6 +158,15 @@ int ref_inc_d(int L, long count, const double *a, const double *y, double *c)
         ref_inc_d1(L, t, y[i]);
         std::memcpy(c + i * L, t, sizeof(double) * L);
     }
+    return 0;
+}
+
+int ref_op(int op, int L, long count, const double *a, const double *b, double *c)
+{
+    if (!ok_L(L) || op < 0 || op > 10) return 1;
+    const int bstride = (op == 6 || op == 7 || op == 8) ? 1 : (op <= 3 ? L : 0);
+    for (long i = 0; i < count; ++i)
+        if (ref_op1(op, L, a + i * L, b ? b + i * bstride : nullptr, c + i * L)) return 1;
     return 0;
 }
 

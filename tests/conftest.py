@@ -50,6 +50,21 @@ def assert_bits_equal(got, want, what=""):
                              f"got {np.asarray(got).ravel()[i]!r} want {np.asarray(want).ravel()[i]!r}")
 
 
+def assert_bits_equal_nan(got, want, what=""):
+    """Bitwise equality, except that any NaN matches any NaN: the sign and
+    payload of a NaN made by 0/0 or Inf - Inf depend on the compiler's
+    operand order and on the hardware, not on the algorithm."""
+    g = np.array(got, dtype=np.float64).ravel()
+    w = np.array(want, dtype=np.float64).ravel()
+    gn, wn = np.isnan(g), np.isnan(w)
+    if not np.array_equal(gn, wn):
+        i = int(np.nonzero(gn != wn)[0][0])
+        raise AssertionError(f"{what}: NaN at different positions; first at {i}: got {g[i]!r} want {w[i]!r}")
+    g[gn] = 0.0
+    w[wn] = 0.0
+    assert_bits_equal(g, w, what)
+
+
 def to_host(t):
     return t.detach().cpu().numpy()
 

@@ -70,5 +70,21 @@ int main(int argc, char **argv)
     dump(dir, "part1", part(A, p, 1));
     dump(dir, "bigA", bigA(A, n, p));
     dump(dir, "bigB", bigB(B, n, p));
+    // the per-(p, pp) squeeze names on the parts the splitters make
+    dump(dir, "sq_p", pllsqueeze(C1q, p, pp));
+    dump(dir, "sqnt_p", pllntsqueeze(C1q, p, pp));
+    if (p == 2) {
+        std::vector<double> x8 = split4pd(A, 2), x16 = split8pd(A, 2), x12 = mixsplit2(A);
+        dump(dir, "x8", x8);
+        dump(dir, "x12", x12);
+        dump(dir, "x16", x16);
+        dump(dir, "sq_2_8", pllsqueeze_2_8(x8));
+        dump(dir, "sq_2_12", pllsqueeze_2_12(x12));
+        dump(dir, "sq_2_16", pllsqueeze_2_16(x16));
+        dump(dir, "sqnt_2_8", pllntsqueeze_2_8(x8));
+        dump(dir, "sqnt_2_12", pllntsqueeze_2_12(x12));
+        dump(dir, "sqnt_2_16", pllntsqueeze_2_16(x16));
+        dump(dir, "sq_12", pllsqueeze(x12, 2, 12));
+    }
     return 0;
 }

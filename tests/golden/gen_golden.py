@@ -51,6 +51,16 @@ def main():
         out[f"L{L}_mul"] = O.op_mul(L, a, b, use_ref=True)
         out[f"L{L}_add"] = O.op_add(L, a, b, use_ref=True)
         out[f"L{L}_inc_d"] = O.op_inc_d(L, a, y, use_ref=True)
+        # the scalar operations off the GEMM path
+        sq = (a.reshape(-1, L) * np.where(a.reshape(-1, L)[:, :1] < 0, -1.0, 1.0)).ravel()
+        pw = np.ldexp(1.0, np.arange(a.size // L) % 41 - 20)
+        out[f"L{L}_sqrt_x"], out[f"L{L}_pw"] = sq, pw
+        for name, code, x, bb in (("sub", O.OP_SUB, a, b), ("div", O.OP_DIV, a, b),
+                                  ("sqr", O.OP_SQR, a, None), ("sqrt", O.OP_SQRT, sq, None),
+                                  ("sqrt_signed", O.OP_SQRT, a, None),
+                                  ("mul_d", O.OP_MUL_D, a, y), ("mul_pwr2", O.OP_MUL_PWR2, a, pw),
+                                  ("abs", O.OP_ABS, a, None), ("neg", O.OP_NEG, a, None)):
+            out[f"L{L}_op_{name}"] = O.op(code, L, x, bb, use_ref=True)
         if L > 2:
             f = op_inputs(L + 1, 64, 400 + L)
             out[f"L{L}_f"] = f

@@ -45,3 +45,15 @@ def test_compat_driver(orc, tmp_path, p):
         for j in range(i + 1):
             want[i, :, j, :] = B3[:, :, i - j].T
     assert_bits_equal(ld["bigB"], want.ravel(), "bigB")
+    want_sq = orc.squeeze(p, pp, C1q)
+    assert_bits_equal(ld["sq_p"], want_sq, "pllsqueeze")
+    assert_bits_equal(ld["sqnt_p"], want_sq, "pllntsqueeze")
+    if p == 2:
+        assert_bits_equal(ld["x8"], orc.split4pd(A, 2), "split4pd")
+        assert_bits_equal(ld["x16"], orc.split8pd(A, 2), "split8pd")
+        assert_bits_equal(ld["x12"], orc.mixsplit2(A), "mixsplit2")
+        for pp2 in (8, 12, 16):
+            want2 = orc.squeeze(2, pp2, ld[f"x{pp2}"])
+            assert_bits_equal(ld[f"sq_2_{pp2}"], want2, f"pllsqueeze_2_{pp2}")
+            assert_bits_equal(ld[f"sqnt_2_{pp2}"], want2, f"pllntsqueeze_2_{pp2}")
+        assert_bits_equal(ld["sq_12"], orc.squeeze(2, 12, ld["x12"]), "pllsqueeze(x, 2, 12)")
