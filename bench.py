@@ -11,6 +11,12 @@ against the exact checker is reported beside its throughput.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  Effective GFLOP/s = 2 m n k / time.
+
+    python bench.py --elementwise [--ew-entries N]
+
+instead times the elementwise p-double operations (mdg_elementwise) on N
+entries per limb count and prints one JSON line of entries/s and GB/s per
+operation, each result sampled against the oracle bit for bit.
 """
 import argparse
 import importlib
@@ -52,7 +58,60 @@ def parse():
                          "modulo coprime moduli (CRT), f64_residue = the same on FP64 MFMA with 21-bit "
                          "primes, f64 = FP64-MFMA split, bf16 = BF16 slices, "
                          "vector = one thread per output entry")
+    ap.add_argument("--elementwise", action="store_true",
+                    help="time the elementwise operations instead of the GEMM")
+    ap.add_argument("--ew-entries", type=int, default=1 << 22)
     return ap.parse_args()
+
+
+def bench_elementwise(a):
+    """Throughput of mdg_elementwise: every op at L = 2, 4, 8, 16 on
+    `ew_entries` entries (e in [-20, 20]), FMA two_prod, inputs in HBM;
+    bytes are the algorithmic operand and result bytes per entry."""
+    import numpy as np
+    import torch
+    import pyoracle as O
+
+    mdg = importlib.import_module(PKG)
+    torch.cuda.set_device(0)
+    n = a.ew_entries
+    ops = ("add", "sub", "mul", "div", "sqr", "sqrt", "mul_d", "inc_d", "neg")
+    out = {}
+    for L in (2, 4, 8, 16):
+        x = mdg.fill_random(mdg.GEN_QD, L, n, 1, 11, -20, 20)
+        y = mdg.fill_random(mdg.GEN_QD, L, n, 1, 12, -20, 20)
+        d = mdg.fill_random(mdg.GEN_QD, 1, n, 1, 13, -20, 20)
+        xs = x.abs()   # positive limbs: roots of positive numbers
+        c = mdg.empty(n * L)
+        stream = torch.cuda.current_stream()
+        for op in ops:
+            xa = xs if op == "sqrt" else x
+            yb = y if op in mdg.BINARY_OPS else (d if op in mdg.DOUBLE_OPS else None)
+            mdg.elementwise(op, L, xa, yb, out=c)
+            torch.cuda.synchronize()
+            reps = 10 if (L < 8 or op in ("add", "sub", "neg", "inc_d", "mul_d")) else 3
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                mdg.elementwise(op, L, xa, yb, out=c, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            nbytes = L * 8 * (3 if op in mdg.BINARY_OPS else 2) + (8 if op in mdg.DOUBLE_OPS else 0)
+            # the first 512 entries against the oracle
+            cnt = 512
+            xh = xa[:cnt * L].cpu().numpy()
+            yh = None if yb is None else yb[:cnt * (L if op in mdg.BINARY_OPS else 1)].cpu().numpy()
+            want = O.op(O.__dict__["OP_" + op.upper()], L, xh, yh, mode=O.TP_FMA)
+            got = c[:cnt * L].cpu().numpy()
+            ok = bool(np.array_equal(np.isnan(got), np.isnan(want)) and np.array_equal(
+                np.nan_to_num(got).view(np.uint64), np.nan_to_num(want).view(np.uint64)))
+            out[f"{op}_L{L}"] = {"ms": round(ms, 4), "Gentries_per_s": round(n / ms / 1e6, 3),
+                                 "GB_per_s": round(n * nbytes / ms / 1e6, 1), "bit_exact": ok}
+            log(f"{op} L={L}: {out[f'{op}_L{L}']}")
+    print(json.dumps({"metric": "elementwise_throughput", "unit": "Gentries/s", "entries": n,
+                      "twoprod": "fma", "hbm_peak_GB_per_s": 8000, "ops": out}))
 
 
 def log(msg):
@@ -76,6 +135,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
               "sharding is described in DESIGN.md and not built.", file=sys.stderr)
         sys.exit(2)
+    if a.elementwise:
+        bench_elementwise(a)
+        return
     import numpy as np
     import torch
 

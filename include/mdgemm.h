@@ -251,6 +251,22 @@ int mdg_elementwise(int op, int L, long long n, const double *a, const double *b
  * Utilities.
  * -------------------------------------------------------------------- */
 
+/* Limb-planar copy of an interleaved rows x cols matrix (row stride ldx
+   entries): planar[l * rows * cols + r * cols + c] = X[(r * ldx + c) * L + l],
+   every limb's part() (C++/TCcuda.cu:34-40) at once.  The buffers must not
+   overlap. */
+int mdg_to_planar(int L, int rows, int cols, const double *x, long long ldx, double *planar, void *stream);
+
+/* The inverse of mdg_to_planar. */
+int mdg_from_planar(int L, int rows, int cols, const double *planar, double *x, long long ldx, void *stream);
+
+/* Y = X^T for an interleaved rows x cols matrix X (row stride ldx); Y is
+   cols x rows with row stride ldy >= rows.  The GEMMs take row strides but
+   no transposes (the reference is square and row-major only): a transposed
+   operand is materialised with this first.  The buffers must not overlap. */
+int mdg_transpose(int L, int rows, int cols, const double *x, long long ldx, double *y, long long ldy,
+                  void *stream);
+
 /* Seeded counter-based generator: `count` L-limb entries.
    kind 0: hi ~ U[-1,1), lo = hi*2^-53*U[-0.5,0.5), QuickTwoSum(hi, lo)
    kind 1: leading limb s*2^e, e uniform in [emin, emax], s ~ +-U[1,2);

@@ -22,6 +22,9 @@ int squeeze(int L, long long nent, int pp, const double *x, double *y, hipStream
 int elementwise(int op, int L, long long n, const double *a, const double *b, double *c, int tp, hipStream_t s);
 bool elementwise_binary(int op);
 bool elementwise_takes_double(int op);
+int to_planar(int L, int rows, int cols, const double *x, long long ldx, double *planar, hipStream_t s);
+int from_planar(int L, int rows, int cols, const double *planar, double *x, long long ldx, hipStream_t s);
+int transpose(int L, int rows, int cols, const double *x, long long ldx, double *y, long long ldy, hipStream_t s);
 int bigB2(const double *Bq, int n, int pp, double *out, hipStream_t s);
 int fill_random(int kind, int L, long long count, unsigned long long seed, int emin, int emax, double *out,
                 hipStream_t s);
@@ -331,6 +334,47 @@ int mdg_elementwise(int op, int L, long long n, const double *a, const double *b
     if (rc) return rc;
     if (n == 0) return MDG_SUCCESS;
     return elementwise(op, L, n, a, b, c, twoprod, as_stream(stream));
+}
+
+static int check_layout_args(int L, int rows, int cols, const void *x, long long ldx, const void *y)
+{
+    MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d", L);
+    MDG_REQUIRE(rows >= 0 && cols >= 0, MDG_ERROR_INVALID_VALUE, "bad shape %d x %d", rows, cols);
+    MDG_REQUIRE(rows == 0 || cols == 0 || ldx >= cols, MDG_ERROR_INVALID_VALUE, "row stride %lld < cols %d", ldx,
+                cols);
+    MDG_REQUIRE(rows == 0 || cols == 0 || (x && y), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    MDG_REQUIRE((long long)((rows + 15) / 16) * ((cols + 15) / 16) < (1ll << 31) &&
+                    (long long)rows * ((cols + 255) / 256) < (1ll << 31),
+                MDG_ERROR_INVALID_VALUE, "matrix %d x %d too large", rows, cols);
+    return MDG_SUCCESS;
+}
+
+int mdg_to_planar(int L, int rows, int cols, const double *x, long long ldx, double *planar, void *stream)
+{
+    int rc = check_layout_args(L, rows, cols, x, ldx, planar);
+    if (rc || (rc = ensure_device())) return rc;
+    if (rows == 0 || cols == 0) return MDG_SUCCESS;
+    return to_planar(L, rows, cols, x, ldx, planar, as_stream(stream));
+}
+
+int mdg_from_planar(int L, int rows, int cols, const double *planar, double *x, long long ldx, void *stream)
+{
+    int rc = check_layout_args(L, rows, cols, x, ldx, planar);
+    if (rc || (rc = ensure_device())) return rc;
+    if (rows == 0 || cols == 0) return MDG_SUCCESS;
+    return from_planar(L, rows, cols, planar, x, ldx, as_stream(stream));
+}
+
+int mdg_transpose(int L, int rows, int cols, const double *x, long long ldx, double *y, long long ldy, void *stream)
+{
+    int rc = check_layout_args(L, rows, cols, x, ldx, y);
+    if (rc) return rc;
+    MDG_REQUIRE(rows == 0 || cols == 0 || ldy >= rows, MDG_ERROR_INVALID_VALUE, "row stride %lld < rows %d", ldy,
+                rows);
+    MDG_REQUIRE(rows == 0 || cols == 0 || x != y, MDG_ERROR_INVALID_VALUE, "in-place transpose is not supported");
+    if ((rc = ensure_device())) return rc;
+    if (rows == 0 || cols == 0) return MDG_SUCCESS;
+    return transpose(L, rows, cols, x, ldx, y, ldy, as_stream(stream));
 }
 
 int mdg_tc_gemm(int L, int m, int n, int k, const double *A, const double *B, double *C, int split_kind,

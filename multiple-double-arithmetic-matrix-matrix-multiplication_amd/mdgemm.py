@@ -75,6 +75,9 @@ _SIGS = {
     "mdg_tc_gemm": ([_i, _i, _i, _i, _vp, _vp, _vp, _i, _vp], _i),
     "mdg_fill_random": ([_i, _i, _ll, _ull, _i, _i, _vp, _vp], _i),
     "mdg_elementwise": ([_i, _i, _ll, _vp, _vp, _vp, _i, _vp], _i),
+    "mdg_to_planar": ([_i, _i, _i, _vp, _ll, _vp, _vp], _i),
+    "mdg_from_planar": ([_i, _i, _i, _vp, _vp, _ll, _vp], _i),
+    "mdg_transpose": ([_i, _i, _i, _vp, _ll, _vp, _ll, _vp], _i),
 }
 
 
@@ -246,4 +249,29 @@ def elementwise(op, L, a, b=None, twoprod=TWOPROD_FMA, out=None, stream=None):
     if out is None:
         out = empty(n * L)
     _check(lib().mdg_elementwise(code, L, n, _ptr(a), _ptr(b), _ptr(out), twoprod, _stream(stream)))
+    return out
+
+
+def to_planar(L, rows, cols, x, ldx=None, stream=None):
+    """Limb planes of an interleaved rows x cols matrix: out[l*rows*cols + r*cols + c]."""
+    out = empty(rows * cols * L)
+    _check(lib().mdg_to_planar(L, rows, cols, _ptr(x), cols if ldx is None else ldx, _ptr(out),
+                               _stream(stream)))
+    return out
+
+
+def from_planar(L, rows, cols, planar, out=None, ldx=None, stream=None):
+    ldx = cols if ldx is None else ldx
+    if out is None:
+        out = empty(rows * ldx * L)
+    _check(lib().mdg_from_planar(L, rows, cols, _ptr(planar), _ptr(out), ldx, _stream(stream)))
+    return out
+
+
+def transpose(L, rows, cols, x, ldx=None, ldy=None, stream=None):
+    """The cols x rows transpose of an interleaved rows x cols matrix."""
+    ldy = rows if ldy is None else ldy
+    out = empty(cols * ldy * L)
+    _check(lib().mdg_transpose(L, rows, cols, _ptr(x), cols if ldx is None else ldx, _ptr(out), ldy,
+                               _stream(stream)))
     return out
