@@ -11,8 +11,12 @@
 //                 pipeline is recorded once into a hipGraph keyed by the
 //                 batch's shape signature and replayed on later calls.
 // Slice counts come from the caller's spread allowance (no device->host
-// read), so nothing in the batch synchronises the stream.  Methods other
-// than INT8 and the vector GEMM run the per-problem loop inside the graph.
+// read), so nothing in the batch synchronises the stream.  The INT8
+// residue method shares the scans and continues with the fused residue
+// stages of residue_gemm.hip (residues, one launch of INT8 MFMA tiles of all
+// residue planes, CRT reconstruction).  Other methods, and residue batches
+// with a problem deeper than one INT32 plane allows, run the per-problem
+// loop inside the graph.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -24,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "i8tile.hpp"
 #include "md_arith.hpp"
 #include "recon.hpp"
 #include "slices.hpp"
@@ -35,8 +40,12 @@ int vector_gemm(int L, int m, int n, int k, const double *A, long long lda, cons
 int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
                long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s);
 int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b);
+struct ResBatch;
+int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *const *F, int target_bits, int spread,
+                        ResBatch **out);
+int residue_batch_run(const ResBatch *rb, hipStream_t s);
+void residue_batch_free(ResBatch *rb);
 
-typedef int i4 __attribute__((ext_vector_type(4)));
 constexpr int BZERO_SCALE = -4000;
 
 // one problem of the fused pipeline, as the kernels see it
@@ -186,9 +195,8 @@ __device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 
 
 __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
-    constexpr int BM = 128, BN = 128, BK = 128, LDR = BK + 16;
-    __shared__ __attribute__((aligned(16))) int8_t As[BM * LDR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[BN * LDR];
+    __shared__ __attribute__((aligned(16))) int8_t As[I8T_BM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[I8T_BM * I8T_LDR];
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
     const int S = p.S, kp = p.kp, kmax = p.kmax;
@@ -201,72 +209,12 @@ __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict_
     const int c = e.z - first;
     const int k0 = c * kmax, k1 = min((g + 1) * kp, k0 + kmax);
     const long long ld = (long long)S * kp;
-    const int8_t *Ab = p.Aall + (long long)e.x * BM * ld + k0;
-    const int8_t *Bb = p.Ball + (long long)e.y * BN * ld + (long long)(S - 1 - g) * kp + k0;
-    const int Kd = k1 - k0;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int lrow = lane & 15, lk = (lane >> 4) * 16;
-    i4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (i4){0, 0, 0, 0};
-    const int frow = tid >> 3, fseg = (tid & 7) * 16;
-    const long long foff = (long long)frow * ld + fseg, fstep = 32 * ld;
-    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
-    i4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define MDG_BFETCH(kb)                                       \
-    do {                                                     \
-        ra0 = *(const i4 *)(Ab + foff + (kb));               \
-        ra1 = *(const i4 *)(Ab + foff + fstep + (kb));       \
-        ra2 = *(const i4 *)(Ab + foff + 2 * fstep + (kb));   \
-        ra3 = *(const i4 *)(Ab + foff + 3 * fstep + (kb));   \
-        rb0 = *(const i4 *)(Bb + foff + (kb));               \
-        rb1 = *(const i4 *)(Bb + foff + fstep + (kb));       \
-        rb2 = *(const i4 *)(Bb + foff + 2 * fstep + (kb));   \
-        rb3 = *(const i4 *)(Bb + foff + 3 * fstep + (kb));   \
-    } while (0)
-    MDG_BFETCH(0);
-    for (int kb = 0; kb < Kd; kb += BK) {
-        *(i4 *)&As[soff] = ra0;
-        *(i4 *)&As[soff + sstep] = ra1;
-        *(i4 *)&As[soff + 2 * sstep] = ra2;
-        *(i4 *)&As[soff + 3 * sstep] = ra3;
-        *(i4 *)&Bs[soff] = rb0;
-        *(i4 *)&Bs[soff + sstep] = rb1;
-        *(i4 *)&Bs[soff + 2 * sstep] = rb2;
-        *(i4 *)&Bs[soff + 3 * sstep] = rb3;
-        __syncthreads();
-        if (kb + BK < Kd) MDG_BFETCH(kb + BK);
-#pragma unroll
-        for (int ks = 0; ks < BK; ks += 64) {
-            i4 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = *(const i4 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = *(const i4 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#undef MDG_BFETCH
-    int *plane = p.D + (long long)e.z * p.m * p.n;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long col = (long long)e.y * BN + wn * 64 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const long long row = (long long)e.x * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
-                if (row < p.m && col < p.n) plane[row * p.n + col] = acc[i][j][v];
-            }
-        }
+    const int8_t *Ab = p.Aall + (long long)e.x * I8T_BM * ld + k0;
+    const int8_t *Bb = p.Ball + (long long)e.y * I8T_BM * ld + (long long)(S - 1 - g) * kp + k0;
+    i4t acc[4][4];
+    i8_tile_128(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
+    i8_tile_store(acc, p.D + (long long)e.z * p.m * p.n, p.n, (long long)e.x * I8T_BM, (long long)e.y * I8T_BM, p.m,
+                  p.n);
 }
 
 // ------------------------------------------------------- reconstruction --
@@ -380,9 +328,11 @@ struct Plan {
     hipGraphExec_t exec = nullptr;
     Stage gemm_alt{6, 0, 0, 0};   // problem-major order of the GEMM tiles
     bool use_alt = false;         // chosen by timing both orders once
+    ResBatch *rb = nullptr;       // INT8 residue stages after the scans
     ~Plan()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
+        residue_batch_free(rb);
         (void)hipFree(dprobs);
         (void)hipFree(dents);
         (void)hipFree(arena);
@@ -392,6 +342,7 @@ struct Plan {
 static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, int target_bits, int spread)
 {
     const bool i8 = (method == MDG_METHOD_INT8_OZAKI);
+    const bool res = (method == MDG_METHOD_INT8_RESIDUE);   // scans here, the rest in residue_gemm.hip
     // arena layout
     size_t bytes = 0;
     auto take = [&](size_t b) {
@@ -418,7 +369,15 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         b.A = q.A;
         b.B = q.B;
         b.C = q.C;
-        if (!i8 || q.m == 0 || q.n == 0 || q.k == 0) continue;
+        if (q.m == 0 || q.n == 0 || q.k == 0) continue;
+        if (res) {
+            offs[i].e = take((size_t)b.m * 4);
+            offs[i].f = take((size_t)b.n * 4);
+            offs[i].cs = take((size_t)b.n * 8);
+            b.S = -1;   // marks a problem with scale exponents
+            continue;
+        }
+        if (!i8) continue;
         b.w = 8;
         b.S = plan_int8_slices(q.L, target_bits > 0 ? target_bits : 53 * q.L, spread, spread);
         MDG_REQUIRE(b.S > 0 && b.S * 8 <= 32 * (q.L == 2 ? 8 : q.L == 4 ? 12 : q.L == 8 ? 20 : 36) - 1,
@@ -442,9 +401,11 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         for (int i = 0; i < count; ++i) {
             BProb &b = pl.probs[i];
             if (!b.S) continue;
-            b.Aall = (int8_t *)(base + offs[i].aall);
-            b.Ball = (int8_t *)(base + offs[i].ball);
-            b.D = (int *)(base + offs[i].d);
+            if (b.S > 0) {
+                b.Aall = (int8_t *)(base + offs[i].aall);
+                b.Ball = (int8_t *)(base + offs[i].ball);
+                b.D = (int *)(base + offs[i].d);
+            }
             b.E = (int *)(base + offs[i].e);
             b.F = (int *)(base + offs[i].f);
             b.colsum = (double *)(base + offs[i].cs);
@@ -457,7 +418,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
             const BProb &b = pl.probs[i];
             if (L && b.L != L) continue;
             if (b.m == 0 || b.n == 0 || b.k == 0) continue;
-            if (i8 && !b.S) continue;
+            if ((i8 || res) && !b.S) continue;
             gen(i, b);
         }
         st.count = (long long)pl.ents.size() - st.first;
@@ -465,7 +426,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
     };
     auto push = [&](int i, int x, int y, int z) { pl.ents.push_back(BEnt{i, x, y, z}); };
     const int Ls[4] = {2, 4, 8, 16};
-    if (i8) {
+    if (i8 || res) {
         add_stage(0, 0, [&](int i, const BProb &b) {
             for (int x = 0; x < (b.n + 255) / 256; ++x) push(i, x, 0, 0);
         });
@@ -481,6 +442,17 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         add_stage(3, 0, [&](int i, const BProb &b) {
             for (int x = 0; x < (b.n + 255) / 256; ++x) push(i, x, 0, 0);
         });
+    }
+    if (res) {
+        std::vector<int *> Es(count, nullptr), Fs(count, nullptr);
+        for (int i = 0; i < count; ++i) {
+            Es[i] = pl.probs[i].E;
+            Fs[i] = pl.probs[i].F;
+        }
+        const int rc = residue_batch_build(count, pr, Es.data(), Fs.data(), target_bits, spread, &pl.rb);
+        if (rc) return rc;
+    }
+    if (i8) {
         for (int L : Ls)
             add_stage(4, L, [&](int i, const BProb &b) {
                 const long long nb = ((long long)b.mp * b.kp + 255) / 256;
@@ -530,7 +502,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                         pl.stages.back().chunked = pl.stages.back().chunked || b.nplanes > b.S;
                     }
         }
-    } else {
+    } else if (!res) {
         for (int L : Ls)
             add_stage(8, L, [&](int i, const BProb &b) {
                 for (int x = 0; x < (b.m + 15) / 16; ++x)
@@ -598,7 +570,7 @@ static int run_plan(const Plan &pl, hipStream_t s)
         int rc = launch_stage(pl, (st.kind == 6 && pl.use_alt) ? pl.gemm_alt : st, s);
         if (rc) return rc;
     }
-    return MDG_SUCCESS;
+    return pl.rb ? residue_batch_run(pl.rb, s) : MDG_SUCCESS;
 }
 
 // Autotune the batch schedule: run the pipeline with each GEMM tile order and
@@ -673,8 +645,17 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         return MDG_SUCCESS;
     }
     Plan *pl = new Plan();
-    const bool fused = (method == MDG_METHOD_INT8_OZAKI || method == MDG_METHOD_VECTOR);
+    bool fused =
+        (method == MDG_METHOD_INT8_OZAKI || method == MDG_METHOD_VECTOR || method == MDG_METHOD_INT8_RESIDUE);
     int rc = fused ? build_plan(*pl, count, pr, method, target_bits, spread) : MDG_SUCCESS;
+    if (rc == MDG_ERROR_NOT_SUPPORTED && method == MDG_METHOD_INT8_RESIDUE) {
+        // a problem too deep for the fused residue stages: the per-problem
+        // calls inside the graph instead
+        delete pl;
+        pl = new Plan();
+        fused = false;
+        rc = MDG_SUCCESS;
+    }
     if (rc) {
         delete pl;
         return rc;

@@ -29,6 +29,7 @@
 // W = target + spread_a + spread_b + 3 + log2(L).
 #include <climits>
 #include <cmath>
+#include <cstring>
 
 #include <algorithm>
 #include <map>
@@ -36,6 +37,7 @@
 #include <tuple>
 #include <vector>
 
+#include "i8tile.hpp"
 #include "md_arith.hpp"
 #include "slices.hpp"
 #include "split_common.hpp"
@@ -208,6 +210,22 @@ __device__ __forceinline__ void entry_residues(const double (&x)[EPT][L], int sh
     }
 }
 
+// the residues of EPT zero entries (padding, zero rows and columns): zero
+// words in every plane, in the layout of entry_residues
+template <int EPT>
+__device__ __forceinline__ void zero_residues(const ResTab &t, int8_t *o8, long long plane8, int8_t *o16,
+                                              long long plane16, int kp, bool triple)
+{
+    typedef typename PackOf<EPT>::T PT;
+    for (int i = 0; i < t.n8; ++i) *(PT *)(o8 + i * plane8) = 0;
+    for (int j = 0; j < t.n16; ++j) {
+        int8_t *o = o16 + j * plane16;
+        *(PT *)o = 0;
+        *(PT *)(o + kp) = 0;
+        if (triple) *(PT *)(o + 2 * kp) = 0;
+    }
+}
+
 // entries per thread: packed stores of 4/2/1 bytes per plane, within the
 // register budget of the fixed-point words and chunks
 template <int L>
@@ -219,12 +237,11 @@ struct ResEPT {
 // mp x kp domain.  Narrow planes R8[i][r][kk] (mp x kp), wide planes
 // R16[j][r][2kp]: hi at kk, lo at kp + kk.
 template <int L, int NWU>
-__global__ __launch_bounds__(256) void residue_a_kernel(int m, int k, int mp, int kp, const double *__restrict__ A,
-                                                        long long lda, const int *__restrict__ E, int W, ResTab t,
-                                                        int8_t *__restrict__ R8, int8_t *__restrict__ R16)
+__device__ __forceinline__ void residue_a_at(long long idx, int m, int k, int mp, int kp, const double *__restrict__ A,
+                                             long long lda, const int *__restrict__ E, int W, const ResTab &t,
+                                             int8_t *__restrict__ R8, int8_t *__restrict__ R16)
 {
     constexpr int EPT = ResEPT<L>::value;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int kq = kp / EPT;
     if (idx >= (long long)mp * kq) return;
     const long long r = idx / kq;
@@ -232,13 +249,25 @@ __global__ __launch_bounds__(256) void residue_a_kernel(int m, int k, int mp, in
     double x[EPT][L];
     int shift = 0;
     const bool live = r < m && E[r] != ZERO_SCALE;
-    if (live) shift = W - 2 - E[r];
+    if (!live || kk >= k) {
+        zero_residues<EPT>(t, R8 + r * kp + kk, (long long)mp * kp, R16 + r * 2 * kp + kk, 2ll * mp * kp, kp, false);
+        return;
+    }
+    shift = W - 2 - E[r];
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
 #pragma unroll
-        for (int l = 0; l < L; ++l) x[e][l] = (live && kk + e < k) ? A[(r * lda + kk + e) * L + l] : 0.0;
+        for (int l = 0; l < L; ++l) x[e][l] = (kk + e < k) ? A[(r * lda + kk + e) * L + l] : 0.0;
     entry_residues<L, NWU, EPT>(x, shift, t, R8 + r * kp + kk, (long long)mp * kp, R16 + r * 2 * kp + kk,
                                 2ll * mp * kp, kp, false);
+}
+
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void residue_a_kernel(int m, int k, int mp, int kp, const double *__restrict__ A,
+                                                        long long lda, const int *__restrict__ E, int W, ResTab t,
+                                                        int8_t *__restrict__ R8, int8_t *__restrict__ R16)
+{
+    residue_a_at<L, NWU>((long long)blockIdx.x * blockDim.x + threadIdx.x, m, k, mp, kp, A, lda, E, W, t, R8, R16);
 }
 
 // B: a block stages a RES_TK x TC tile of B (rows kk, columns c; coalesced
@@ -246,10 +275,11 @@ __global__ __launch_bounds__(256) void residue_a_kernel(int m, int k, int mp, in
 // so the transposed residue writes coalesce along kk.  Narrow planes
 // R8[i][c][kk] (np x kp), wide planes R16[j][c][3kp]: hi at kk and
 // 2kp + kk, lo at kp + kk -- [kp, 3kp) is (lo, hi) against A's (hi, lo).
+// blk numbers the (np / TC) x (kp / RES_TK) tiles.
 template <int L, int NWU>
-__global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, int np, const double *__restrict__ B,
-                                                        long long ldb, const int *__restrict__ F, int W, ResTab t,
-                                                        int8_t *__restrict__ R8, int8_t *__restrict__ R16)
+__device__ __forceinline__ void residue_b_at(long long blk, int k, int n, int kp, int np, const double *__restrict__ B,
+                                             long long ldb, const int *__restrict__ F, int W, const ResTab &t,
+                                             int8_t *__restrict__ R8, int8_t *__restrict__ R16)
 {
     constexpr int EPT = ResEPT<L>::value;
     constexpr int TPC = RES_TK / EPT;   // threads per column
@@ -257,8 +287,8 @@ __global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, in
     constexpr int ROW = TC * L + 1;     // padded LDS row, doubles
     __shared__ double tile[RES_TK * ROW];
     const int ktiles = kp / RES_TK;
-    const int c0 = (int)(blockIdx.x / ktiles) * TC;
-    const int k0 = (int)(blockIdx.x % ktiles) * RES_TK;
+    const int c0 = (int)(blk / ktiles) * TC;
+    const int k0 = (int)(blk % ktiles) * RES_TK;
     for (int q = threadIdx.x; q < RES_TK * TC * L; q += 256) {
         const int row = q / (TC * L), col = q - row * (TC * L);
         const int kk = k0 + row, c = c0 + col / L;
@@ -268,14 +298,27 @@ __global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, in
     const int cc = threadIdx.x / TPC, kq = threadIdx.x % TPC;
     const int c = c0 + cc, kk = k0 + kq * EPT;
     const bool live = c < n && F[c] != ZERO_SCALE;
-    const int shift = live ? W - 2 - F[c] : 0;
+    if (!live || kk >= k) {
+        zero_residues<EPT>(t, R8 + (long long)c * kp + kk, (long long)np * kp, R16 + (long long)c * 3 * kp + kk,
+                           3ll * np * kp, kp, true);
+        return;
+    }
+    const int shift = W - 2 - F[c];
     double x[EPT][L];
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
 #pragma unroll
-        for (int l = 0; l < L; ++l) x[e][l] = live ? tile[(kq * EPT + e) * ROW + cc * L + l] : 0.0;
+        for (int l = 0; l < L; ++l) x[e][l] = tile[(kq * EPT + e) * ROW + cc * L + l];
     entry_residues<L, NWU, EPT>(x, shift, t, R8 + (long long)c * kp + kk, (long long)np * kp,
                                 R16 + (long long)c * 3 * kp + kk, 3ll * np * kp, kp, true);
+}
+
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void residue_b_kernel(int k, int n, int kp, int np, const double *__restrict__ B,
+                                                        long long ldb, const int *__restrict__ F, int W, ResTab t,
+                                                        int8_t *__restrict__ R8, int8_t *__restrict__ R16)
+{
+    residue_b_at<L, NWU>(blockIdx.x, k, n, kp, np, B, ldb, F, W, t, R8, R16);
 }
 
 // ----------------------------------------------- FP64-MFMA residue planes --
@@ -399,13 +442,18 @@ struct ResPlanes {
     long long ps;                     // plane stride (elements)
     int nck1, nck2;                   // K-chunks of the depth-k and depth-2k GEMMs
     int reduce;                       // narrow triples: reduce each plane value first
+    // planes already reduced to [0, m) (the fused batch's GEMM epilogue):
+    // narrow i at P8 + i ps; wide j: hh at P16 + j ps, ll at (n16 + j) ps,
+    // cross at (2 n16 + j) ps; one K-chunk
+    const uint8_t *P8 = nullptr;
+    const uint16_t *P16 = nullptr;
 };
 
-template <int L, int NW, bool F64>
-__global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, int W, ResTab t, ResPlanes pl,
-                                                                  const int *__restrict__ E,
-                                                                  const int *__restrict__ F, double *__restrict__ C,
-                                                                  long long ldc)
+template <int L, int NW, bool F64, bool PRE = false>
+__device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int n, int W, const ResTab &t,
+                                                       const ResPlanes &pl, const int *__restrict__ E,
+                                                       const int *__restrict__ F, double *__restrict__ C,
+                                                       long long ldc)
 {
     const int *__restrict__ D8 = pl.D8;
     const int *__restrict__ Dhh = pl.Dhh;
@@ -414,7 +462,6 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
     const long long *__restrict__ D64 = pl.D64;
     const long long ps = pl.ps;
     const int nck1 = pl.nck1, nck2 = pl.nck2, reduce = pl.reduce;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)m * n) return;
     const long long r = idx / n, c = idx - r * n;
     double y[L];
@@ -450,7 +497,10 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
                     const int i = 3 * (g0 + u) + x;
-                    v[u][x] = (i < t.n8) ? D8[(long long)i * nck1 * ps + idx] : 0;
+                    if constexpr (PRE)
+                        v[u][x] = (i < t.n8) ? (int)pl.P8[(long long)i * ps + idx] : 0;
+                    else
+                        v[u][x] = (i < t.n8) ? D8[(long long)i * nck1 * ps + idx] : 0;
                 }
 #pragma unroll
             for (int u = 0; u < RES_GB; ++u) {
@@ -480,9 +530,15 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
             for (int u = 0; u < RES_GB; ++u) {
                 const bool on = j0 + u < t.n16;
                 const long long j = j0 + u;
-                hh[u] = on ? Dhh[j * nck1 * ps + idx] : 0;
-                ll[u] = on ? Dll[j * nck1 * ps + idx] : 0;
-                xx[u] = on ? Dx[j * nck2 * ps + idx] : 0;
+                if constexpr (PRE) {
+                    hh[u] = on ? (int)pl.P16[j * ps + idx] : 0;
+                    ll[u] = on ? (int)pl.P16[(t.n16 + j) * ps + idx] : 0;
+                    xx[u] = on ? (int)pl.P16[(2 * t.n16 + j) * ps + idx] : 0;
+                } else {
+                    hh[u] = on ? Dhh[j * nck1 * ps + idx] : 0;
+                    ll[u] = on ? Dll[j * nck1 * ps + idx] : 0;
+                    xx[u] = on ? Dx[j * nck2 * ps + idx] : 0;
+                }
             }
 #pragma unroll
             for (int u = 0; u < RES_GB; ++u) {
@@ -552,6 +608,16 @@ __global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, 
     double *o = C + (r * ldc + c) * L;
 #pragma unroll
     for (int l = 0; l < L; ++l) o[l] = y[l];
+}
+
+template <int L, int NW, bool F64>
+__global__ __launch_bounds__(256) void residue_reconstruct_kernel(int m, int n, int W, ResTab t, ResPlanes pl,
+                                                                  const int *__restrict__ E,
+                                                                  const int *__restrict__ F, double *__restrict__ C,
+                                                                  long long ldc)
+{
+    residue_reconstruct_at<L, NW, F64>((long long)blockIdx.x * blockDim.x + threadIdx.x, m, n, W, t, pl, E, F, C,
+                                       ldc);
 }
 
 // --------------------------------------------------------------- moduli --
@@ -1175,6 +1241,451 @@ static int run_residue_f64(int m, int n, int k, const double *A, long long lda, 
     }
     return MDG_SUCCESS;
 }
+
+// ------------------------------------------------ fused batch pipeline --
+// Many INT8 residue GEMMs of uneven shape as three launches per limb count
+// (residues of A, of B, reconstruction) plus one launch of the INT8 MFMA
+// tiles of every residue plane of every problem.  Each problem gets the
+// fixed-point width the single-problem path picks for it with the batch's
+// fixed spread; the results are bit-identical to a loop of mdg_gemm calls.
+// The exponent scans (E, F per problem) come from the caller's pipeline.
+
+// one problem, as the kernels see it; residue planes as in run_residue_gemm.
+// Every problem of one limb count uses that limb count's modulus set (the
+// one its deepest problem needs: Z is the same exact integer whatever valid
+// set computes it, so the results stay bit-identical to single calls).
+// Result planes, reduced to [0, m) by the GEMM epilogue: narrow i at
+// P8 + i ps (bytes); wide j: hh at P16 + j ps, ll at (n16 + j) ps, cross at
+// (2 n16 + j) ps (16-bit words).
+struct RProb {
+    int L, m, n, k, mp, np, kp, W, n8, n16;
+    long long lda, ldb, ldc, ps;
+    const double *A, *B;
+    double *C;
+    const int *E, *F;
+    int8_t *A8, *A16, *B8, *B16;
+    uint8_t *P8;
+    uint16_t *P16;
+    const double *md, *inv;   // the moduli of the problem's set and their reciprocals
+};
+
+struct REnt {
+    int prob, x, y, z;
+};
+
+// the limb count's modulus tables come as a kernel argument (read with
+// scalar loads, as in the single-problem kernels)
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void b_residue_a_kernel(const RProb *__restrict__ P, const REnt *__restrict__ T,
+                                                          ResTab t)
+{
+    const REnt e = T[blockIdx.x];
+    const RProb p = P[e.prob];
+    residue_a_at<L, NWU>((long long)e.x * 256 + threadIdx.x, p.m, p.k, p.mp, p.kp, p.A, p.lda, p.E, p.W, t, p.A8,
+                         p.A16);
+}
+
+template <int L, int NWU>
+__global__ __launch_bounds__(256) void b_residue_b_kernel(const RProb *__restrict__ P, const REnt *__restrict__ T,
+                                                          ResTab t)
+{
+    const REnt e = T[blockIdx.x];
+    const RProb p = P[e.prob];
+    residue_b_at<L, NWU>(e.x, p.k, p.n, p.kp, p.np, p.B, p.ldb, p.F, p.W, t, p.B8, p.B16);
+}
+
+// entry (prob, tile row, tile column, plane): plane z < n8 is narrow modulus
+// z; then wide modulus j = (z - n8) / 3 with v = (z - n8) % 3: 0 hi*hi,
+// 1 lo*lo, 2 (hi, lo) . (lo, hi) of depth 2kp.  The epilogue stores each
+// exact INT32 sum reduced modulo its modulus.
+__global__ __launch_bounds__(256) void b_residue_gemm_kernel(const RProb *__restrict__ P, const REnt *__restrict__ T)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[I8T_BM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[I8T_BM * I8T_LDR];
+    const REnt e = T[blockIdx.x];
+    const RProb p = P[e.prob];
+    const long long kp = p.kp;
+    const int8_t *Ab, *Bb;
+    long long lda, ldb;
+    int Kd, mi;
+    if (e.z < p.n8) {
+        Ab = p.A8 + (long long)e.z * p.mp * kp;
+        Bb = p.B8 + (long long)e.z * p.np * kp;
+        lda = ldb = kp;
+        Kd = (int)kp;
+        mi = e.z;
+    } else {
+        const int j = (e.z - p.n8) / 3, v = (e.z - p.n8) % 3;
+        lda = 2 * kp;
+        ldb = 3 * kp;
+        Ab = p.A16 + (long long)j * p.mp * lda + (v == 1 ? kp : 0);
+        Bb = p.B16 + (long long)j * p.np * ldb + (v == 0 ? 0 : kp);
+        Kd = (int)(v == 2 ? 2 * kp : kp);
+        mi = p.n8 + j;
+    }
+    i4t acc[4][4];
+    i8_tile_128(Ab + (long long)e.x * I8T_BM * lda, lda, Bb + (long long)e.y * I8T_BM * ldb, ldb, Kd, As, Bs, acc);
+    const double md = p.md[mi], inv = p.inv[mi];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long long row0 = (long long)e.x * I8T_BM + wm * 64, col0 = (long long)e.y * I8T_BM + wn * 64;
+    if (e.z < p.n8) {
+        uint8_t *plane = p.P8 + (long long)e.z * p.ps;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long col = col0 + j * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const long long row = row0 + i * 16 + (lane >> 4) * 4 + v;
+                    if (row < p.m && col < p.n)
+                        plane[row * p.n + col] = (uint8_t)(int)mod_pos((double)acc[i][j][v], md, inv);
+                }
+            }
+    } else {
+        const int j = (e.z - p.n8) / 3, v3 = (e.z - p.n8) % 3;
+        uint16_t *plane = p.P16 + (long long)(v3 * p.n16 + j) * p.ps;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const long long col = col0 + jj * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const long long row = row0 + i * 16 + (lane >> 4) * 4 + v;
+                    if (row < p.m && col < p.n)
+                        plane[row * p.n + col] = (uint16_t)(int)mod_pos((double)acc[i][jj][v], md, inv);
+                }
+            }
+    }
+}
+
+template <int L, int NW>
+__global__ __launch_bounds__(256) void b_residue_reconstruct_kernel(const RProb *__restrict__ P,
+                                                                    const REnt *__restrict__ T, ResTab t)
+{
+    const REnt e = T[blockIdx.x];
+    const RProb p = P[e.prob];
+    ResPlanes pl{nullptr, nullptr, nullptr, nullptr, nullptr, p.ps, 1, 1, 0};
+    pl.P8 = p.P8;
+    pl.P16 = p.P16;
+    residue_reconstruct_at<L, NW, false, true>((long long)e.x * 256 + threadIdx.x, p.m, p.n, p.W, t, pl, p.E, p.F,
+                                               p.C, p.ldc);
+}
+
+struct ResBatch {
+    struct St {
+        int kind;   // 0 residues of A, 1 of B, 2 GEMM tiles, 3 reconstruction
+        int L, nwu, nw;
+        long long first, count;
+    };
+    std::vector<RProb> probs;
+    std::vector<REnt> ents;
+    std::vector<St> stages;
+    ResTab tab[17] = {};   // per limb count
+    RProb *dprobs = nullptr;
+    REnt *dents = nullptr;
+    void *arena = nullptr;
+    ~ResBatch()
+    {
+        (void)hipFree(dprobs);
+        (void)hipFree(dents);
+        (void)hipFree(arena);
+    }
+};
+
+static int fixed_words(int L)
+{
+    return L == 2 ? FixedWords<2>::NW : L == 4 ? FixedWords<4>::NW : L == 8 ? FixedWords<8>::NW : FixedWords<16>::NW;
+}
+static int res_ept(int L) { return L <= 4 ? 4 : (L == 8 ? 2 : 1); }
+static int nwu_for(int L, int W)
+{
+    switch (L) {
+    case 2: return nwu_bucket<2>(W);
+    case 4: return nwu_bucket<4>(W);
+    case 8: return nwu_bucket<8>(W);
+    default: return nwu_bucket<16>(W);
+    }
+}
+
+// E[i], F[i]: device scale exponents of problem i (filled by the caller's
+// scans before residue_batch_run).  MDG_ERROR_NOT_SUPPORTED when a problem
+// is too deep for one INT32 plane per GEMM (2 kp > RES_KC).
+int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *const *F, int target_bits, int spread,
+                        ResBatch **out)
+{
+    *out = nullptr;
+    ResBatch *rb = new ResBatch();
+    auto fail = [&](int rc) {
+        delete rb;
+        return rc;
+    };
+    // per limb count: fixed-point width and the deepest problem's need
+    int Wl[17] = {0}, needl[17] = {0}, n8l[17] = {0}, n16l[17] = {0}, nwl[17] = {0};
+    for (int i = 0; i < count; ++i) {
+        const mdg_problem &q = pr[i];
+        if (q.m == 0 || q.n == 0 || q.k == 0) continue;
+        const int L = q.L;
+        int lgL = 0, lgk = 0;
+        while ((1 << lgL) < L) ++lgL;
+        while ((1ll << lgk) < q.k) ++lgk;
+        const int tb = target_bits > 0 ? target_bits : 53 * L;
+        const int W = tb + 2 * spread + 3 + lgL;
+        if (W > 32 * fixed_words(L) - 1) {
+            set_error("problem %d needs %d fixed-point bits", i, W);
+            return fail(MDG_ERROR_RANGE);
+        }
+        if (2 * round_up(q.k, 128) > RES_KC) {
+            set_error("problem %d: k=%d too deep for the fused residue batch", i, q.k);
+            return fail(MDG_ERROR_NOT_SUPPORTED);
+        }
+        Wl[L] = W;
+        needl[L] = std::max(needl[L], 2 * W - 1 + lgk);
+    }
+    for (int L : {2, 4, 8, 16}) {
+        if (!needl[L]) continue;
+        int mbits = 0;
+        if (!choose_moduli(needl[L], n8l[L], n16l[L], &mbits)) {
+            set_error("no modulus set reaches %d bits", needl[L]);
+            return fail(MDG_ERROR_RANGE);
+        }
+        nwl[L] = nw_bucket(L, mbits);
+        ResTabHost *th = res_table(n8l[L], n16l[L], 0, nwl[L]);
+        if (!th) {
+            set_error("could not build the modulus tables (%d + %d moduli)", n8l[L], n16l[L]);
+            return fail(MDG_ERROR_HIP);
+        }
+        rb->tab[L] = th->t;
+    }
+    size_t bytes = 0;
+    auto take = [&](size_t b) {
+        const size_t off = bytes;
+        bytes += (b + 255) / 256 * 256;
+        return off;
+    };
+    struct Offs {
+        size_t a8, a16, b8, b16, p8, p16;
+    };
+    std::vector<Offs> offs(count);
+    rb->probs.resize(count);
+    for (int i = 0; i < count; ++i) {
+        const mdg_problem &q = pr[i];
+        RProb &b = rb->probs[i];
+        std::memset(&b, 0, sizeof(b));
+        if (q.m == 0 || q.n == 0 || q.k == 0) continue;
+        const int L = q.L;
+        b.L = L;
+        b.m = q.m;
+        b.n = q.n;
+        b.k = q.k;
+        b.kp = (int)round_up(q.k, 128);
+        b.mp = (int)round_up(q.m, 128);
+        b.np = (int)round_up(q.n, 128);
+        b.W = Wl[L];
+        b.n8 = n8l[L];
+        b.n16 = n16l[L];
+        b.lda = q.lda;
+        b.ldb = q.ldb;
+        b.ldc = q.ldc;
+        b.ps = (long long)q.m * q.n;
+        b.A = q.A;
+        b.B = q.B;
+        b.C = q.C;
+        b.E = E[i];
+        b.F = F[i];
+        b.md = rb->tab[L].md;
+        b.inv = rb->tab[L].inv;
+        offs[i].a8 = take((size_t)b.n8 * b.mp * b.kp);
+        offs[i].a16 = take((size_t)b.n16 * b.mp * 2 * b.kp);
+        offs[i].b8 = take((size_t)b.n8 * b.np * b.kp);
+        offs[i].b16 = take((size_t)b.n16 * b.np * 3 * b.kp);
+        offs[i].p8 = take((size_t)b.n8 * b.ps);
+        offs[i].p16 = take((size_t)3 * b.n16 * b.ps * 2);
+    }
+    if (bytes && hipMalloc(&rb->arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("could not allocate %zu bytes for the residue batch", bytes);
+        return fail(MDG_ERROR_OUT_OF_MEMORY);
+    }
+    char *base = (char *)rb->arena;
+    for (int i = 0; i < count; ++i) {
+        RProb &b = rb->probs[i];
+        if (!b.m) continue;
+        b.A8 = (int8_t *)(base + offs[i].a8);
+        b.A16 = (int8_t *)(base + offs[i].a16);
+        b.B8 = (int8_t *)(base + offs[i].b8);
+        b.B16 = (int8_t *)(base + offs[i].b16);
+        b.P8 = (uint8_t *)(base + offs[i].p8);
+        b.P16 = (uint16_t *)(base + offs[i].p16);
+    }
+    // entry tables
+    auto &ents = rb->ents;
+    auto stage = [&](int kind, int L, auto gen) {
+        ResBatch::St st{kind, L, L ? nwu_for(L, Wl[L]) : 0, L ? nwl[L] : 0, (long long)ents.size(), 0};
+        for (int i = 0; i < count; ++i) {
+            const RProb &b = rb->probs[i];
+            if (!b.m || (L && b.L != L)) continue;
+            gen(i, b);
+        }
+        st.count = (long long)ents.size() - st.first;
+        if (st.count) rb->stages.push_back(st);
+    };
+    const int Ls[4] = {2, 4, 8, 16};
+    for (int L : Ls)
+        stage(0, L, [&](int i, const RProb &b) {
+            const long long nb = ((long long)b.mp * (b.kp / res_ept(L)) + 255) / 256;
+            for (long long x = 0; x < nb; ++x) ents.push_back(REnt{i, (int)x, 0, 0});
+        });
+    for (int L : Ls)
+        stage(1, L, [&](int i, const RProb &b) {
+            const int tc = 256 / (RES_TK / res_ept(L));
+            const long long nb = (long long)(b.np / tc) * (b.kp / RES_TK);
+            for (long long x = 0; x < nb; ++x) ents.push_back(REnt{i, (int)x, 0, 0});
+        });
+    // GEMM tiles of every plane of every problem in one launch.  Workgroup
+    // b runs on XCD b % 8, and each XCD has its own L2: the tiles of one
+    // (problem, plane) go to one XCD, so the tiles that re-read a residue
+    // row block or column block find it in that XCD's L2.  Planes are dealt
+    // to the XCD with the least work so far (deep cross planes first), and
+    // the eight lists are interleaved.
+    {
+        struct Grp {
+            int prob, z, tiles;
+            long long work;
+        };
+        std::vector<Grp> groups;
+        for (int i = 0; i < count; ++i) {
+            const RProb &b = rb->probs[i];
+            if (!b.m) continue;
+            const int tiles = (b.mp / I8T_BM) * (b.np / I8T_BM);
+            for (int z = 0; z < b.n8 + 3 * b.n16; ++z) {
+                const bool cross = z >= b.n8 && (z - b.n8) % 3 == 2;
+                groups.push_back(Grp{i, z, tiles, (long long)tiles * b.kp * (cross ? 2 : 1)});
+            }
+        }
+        std::stable_sort(groups.begin(), groups.end(), [](const Grp &a, const Grp &b) { return a.work > b.work; });
+        constexpr int NX = 8;
+        std::vector<REnt> lists[NX];
+        long long load[NX] = {0};
+        for (const Grp &g : groups) {
+            int x = 0;
+            for (int c = 1; c < NX; ++c)
+                if (load[c] < load[x]) x = c;
+            load[x] += g.work;
+            const RProb &b = rb->probs[g.prob];
+            for (int tx = 0; tx < b.mp / I8T_BM; ++tx)
+                for (int ty = 0; ty < b.np / I8T_BM; ++ty) lists[x].push_back(REnt{g.prob, tx, ty, g.z});
+        }
+        ResBatch::St st{2, 0, 0, 0, (long long)ents.size(), 0};
+        size_t longest = 0;
+        for (const auto &l : lists) longest = std::max(longest, l.size());
+        for (size_t i = 0; i < longest; ++i)
+            for (int x = 0; x < NX; ++x)
+                if (i < lists[x].size()) ents.push_back(lists[x][i]);
+        st.count = (long long)ents.size() - st.first;
+        if (st.count) rb->stages.push_back(st);
+    }
+    for (int L : Ls)
+        stage(3, L, [&](int i, const RProb &b) {
+            const long long nb = ((long long)b.m * b.n + 255) / 256;
+            for (long long x = 0; x < nb; ++x) ents.push_back(REnt{i, (int)x, 0, 0});
+        });
+    for (const auto &st : rb->stages)
+        if (st.count > 2147483647ll) {
+            set_error("residue batch too large for one launch");
+            return fail(MDG_ERROR_NOT_SUPPORTED);
+        }
+    if (hipMalloc(&rb->dprobs, sizeof(RProb) * std::max(1, count)) != hipSuccess ||
+        hipMemcpy(rb->dprobs, rb->probs.data(), sizeof(RProb) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&rb->dents, sizeof(REnt) * std::max<size_t>(1, ents.size())) != hipSuccess ||
+        hipMemcpy(rb->dents, ents.data(), sizeof(REnt) * ents.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("could not upload the residue batch plan");
+        return fail(MDG_ERROR_HIP);
+    }
+    *out = rb;
+    return MDG_SUCCESS;
+}
+
+template <int L>
+static int b_res_launch(const ResBatch::St &st, const ResTab &t, const RProb *P, const REnt *T, hipStream_t s)
+{
+    const dim3 grid((unsigned)st.count), blk(256);
+    if (st.kind == 0 || st.kind == 1) {
+#define MDG_BRES_CASE(NWU)                                                                  \
+    case NWU:                                                                               \
+        if constexpr (NWU <= FixedWords<L>::NW) {                                           \
+            if (st.kind == 0)                                                               \
+                hipLaunchKernelGGL((b_residue_a_kernel<L, NWU>), grid, blk, 0, s, P, T, t); \
+            else                                                                            \
+                hipLaunchKernelGGL((b_residue_b_kernel<L, NWU>), grid, blk, 0, s, P, T, t); \
+            return MDG_SUCCESS;                                                             \
+        }                                                                                   \
+        break;
+        switch (st.nwu) {
+            MDG_BRES_CASE(4)
+            MDG_BRES_CASE(8)
+            MDG_BRES_CASE(12)
+            MDG_BRES_CASE(16)
+            MDG_BRES_CASE(20)
+            MDG_BRES_CASE(24)
+            MDG_BRES_CASE(28)
+            MDG_BRES_CASE(32)
+            MDG_BRES_CASE(36)
+        }
+#undef MDG_BRES_CASE
+        set_error("no residue kernel for %d fixed-point words", st.nwu);
+        return MDG_ERROR_RANGE;
+    }
+#define MDG_BREC_CASE(NW)                                                                  \
+    case NW:                                                                               \
+        hipLaunchKernelGGL((b_residue_reconstruct_kernel<L, NW>), grid, blk, 0, s, P, T, t); \
+        return MDG_SUCCESS;
+    if constexpr (L == 2) {
+        switch (st.nw) { MDG_BREC_CASE(12) MDG_BREC_CASE(16) MDG_BREC_CASE(20) MDG_BREC_CASE(24) }
+    } else if constexpr (L == 4) {
+        switch (st.nw) {
+            MDG_BREC_CASE(16) MDG_BREC_CASE(20) MDG_BREC_CASE(24) MDG_BREC_CASE(28) MDG_BREC_CASE(32)
+            MDG_BREC_CASE(36)
+        }
+    } else if constexpr (L == 8) {
+        switch (st.nw) { MDG_BREC_CASE(32) MDG_BREC_CASE(40) MDG_BREC_CASE(48) MDG_BREC_CASE(56) }
+    } else {
+        switch (st.nw) {
+            MDG_BREC_CASE(56) MDG_BREC_CASE(64) MDG_BREC_CASE(72) MDG_BREC_CASE(80) MDG_BREC_CASE(88)
+            MDG_BREC_CASE(96) MDG_BREC_CASE(104)
+        }
+    }
+#undef MDG_BREC_CASE
+    set_error("no reconstruction kernel for %d CRT words at L=%d", st.nw, L);
+    return MDG_ERROR_RANGE;
+}
+
+int residue_batch_run(const ResBatch *rb, hipStream_t s)
+{
+    for (const auto &st : rb->stages) {
+        const REnt *T = rb->dents + st.first;
+        int rc = MDG_SUCCESS;
+        if (st.kind == 2) {
+            hipLaunchKernelGGL(b_residue_gemm_kernel, dim3((unsigned)st.count), dim3(256), 0, s, rb->dprobs, T);
+        } else {
+            const ResTab &t = rb->tab[st.L];
+            switch (st.L) {
+            case 2: rc = b_res_launch<2>(st, t, rb->dprobs, T, s); break;
+            case 4: rc = b_res_launch<4>(st, t, rb->dprobs, T, s); break;
+            case 8: rc = b_res_launch<8>(st, t, rb->dprobs, T, s); break;
+            default: rc = b_res_launch<16>(st, t, rb->dprobs, T, s); break;
+            }
+        }
+        if (rc) return rc;
+        MDG_LAUNCH_CHECK("residue batch stage");
+    }
+    return MDG_SUCCESS;
+}
+
+void residue_batch_free(ResBatch *rb) { delete rb; }
 
 int residue_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
                  double *C, long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info,

@@ -117,6 +117,22 @@ def test_config4_batched_1024_mixed(mdg, orc, torch_cuda):
         L, m, n, k = p["L"], p["m"], p["n"], p["k"]
         e = orc.normwise_error(L, m, n, k, to_host(p["A"]), to_host(p["B"]), C, nsamp=256)[0]
         assert e <= _bound(L, k)
+    # the same batch through the INT8 residue method: fused pipeline against
+    # its own per-problem loop
+    _, r_loop = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8_residue", persistent=0,
+                                                            spread_bits=16))
+    rloop = [to_host(p["C"]).copy() for p in probs[::64]]
+    _, r_build = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8_residue", persistent=1,
+                                                             spread_bits=16))
+    _, r_replay = _timed(torch_cuda, lambda: mdg.gemm_batched(probs, method="int8_residue", persistent=1,
+                                                              spread_bits=16))
+    print(f"cfg4 INT8 residues: loop {r_loop:.1f} ms, graph build {r_build:.1f} ms, graph replay "
+          f"{r_replay:.1f} ms ({eff / r_replay / 1e6:.1f} GFLOP/s)")
+    for p, C in zip(probs[::64], rloop):
+        assert_bits_equal(to_host(p["C"]), C, "residue persistent vs loop")
+        L, m, n, k = p["L"], p["m"], p["n"], p["k"]
+        e = orc.normwise_error(L, m, n, k, to_host(p["A"]), to_host(p["B"]), C, nsamp=256)[0]
+        assert e <= _bound(L, k)
 
 
 @pytest.fixture(scope="module")
