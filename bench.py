@@ -132,8 +132,8 @@ def read_traffic(kernel, workload):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
-              "sharding is described in DESIGN.md and not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); sharding the GEMM "
+              "by output row blocks over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     if a.elementwise:
         bench_elementwise(a)
