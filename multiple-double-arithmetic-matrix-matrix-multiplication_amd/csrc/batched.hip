@@ -44,6 +44,8 @@ struct ResBatch;
 int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *const *F, int target_bits, int spread,
                         ResBatch **out);
 int residue_batch_run(const ResBatch *rb, hipStream_t s);
+int residue_batch_candidates(const ResBatch *rb);
+void residue_batch_select(ResBatch *rb, int c);
 void residue_batch_free(ResBatch *rb);
 
 constexpr int BZERO_SCALE = -4000;
@@ -188,15 +190,16 @@ __global__ __launch_bounds__(256) void b_slice_b_kernel(const BProb *__restrict_
 }
 
 // --------------------------------------------------- INT8 diagonal tiles --
-// Entry (prob, tile_m, tile_n, plane): a 128 x 128 tile (4 waves, each
-// 64 x 64 of MFMA 16x16x64) of one K-chunk plane of one diagonal.  Plane
-// numbering as in the single-problem path: ascending diagonal g, chunk c.
+// Entry (prob, tile_m, tile_n, plane): a TM x TM tile (i8tile.hpp) of one
+// K-chunk plane of one diagonal.  Plane numbering as in the single-problem
+// path: ascending diagonal g, chunk c.
 __device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
 
+template <int TM>
 __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[I8T_BM * I8T_LDR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[I8T_BM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t As[TM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[TM * I8T_LDR];
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
     const int S = p.S, kp = p.kp, kmax = p.kmax;
@@ -209,12 +212,14 @@ __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict_
     const int c = e.z - first;
     const int k0 = c * kmax, k1 = min((g + 1) * kp, k0 + kmax);
     const long long ld = (long long)S * kp;
-    const int8_t *Ab = p.Aall + (long long)e.x * I8T_BM * ld + k0;
-    const int8_t *Bb = p.Ball + (long long)e.y * I8T_BM * ld + (long long)(S - 1 - g) * kp + k0;
-    i4t acc[4][4];
-    i8_tile_128(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
-    i8_tile_store(acc, p.D + (long long)e.z * p.m * p.n, p.n, (long long)e.x * I8T_BM, (long long)e.y * I8T_BM, p.m,
-                  p.n);
+    const int8_t *Ab = p.Aall + (long long)e.x * TM * ld + k0;
+    const int8_t *Bb = p.Ball + (long long)e.y * TM * ld + (long long)(S - 1 - g) * kp + k0;
+    i4t acc[I8Tile<TM>::IM][I8Tile<TM>::IM];
+    i8_tile<TM>(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
+    int *plane = p.D + (long long)e.z * p.m * p.n;
+    const int n = p.n;
+    i8_tile_each<TM>(acc, (long long)e.x * TM, (long long)e.y * TM, p.m, n,
+                     [&](long long r, long long col, int v) { plane[r * n + col] = v; });
 }
 
 // ------------------------------------------------------- reconstruction --
@@ -316,6 +321,7 @@ struct Stage {
     long long first, count;   // range in the entry table
     int nw = 0;    // reconstruction: diagonal-sum words its problems need
     bool chunked = false;   // reconstruction: some problem has K-chunk planes
+    int tm = 128;  // GEMM: tile rows and columns
 };
 
 struct Plan {
@@ -326,8 +332,10 @@ struct Plan {
     BEnt *dents = nullptr;
     void *arena = nullptr;
     hipGraphExec_t exec = nullptr;
-    Stage gemm_alt{6, 0, 0, 0};   // problem-major order of the GEMM tiles
-    bool use_alt = false;         // chosen by timing both orders once
+    // GEMM stage schedules the autotuner times on the first call: deepest
+    // planes first with 128- and 64-row tiles, problem-major with 128
+    std::vector<Stage> gemm_cands;
+    int gemm_pick = -1;           // -1: the stage as built
     ResBatch *rb = nullptr;       // INT8 residue stages after the scans
     ~Plan()
     {
@@ -463,31 +471,38 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 const long long nb = ((long long)b.np * b.kp + 255) / 256;
                 for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
             });
-        // GEMM tiles: deepest planes of every problem first, so long blocks start early
-        {
+        // GEMM tiles: deepest planes of every problem first, so long blocks
+        // start early (128-row tiles: the stage as built; 64-row tiles: a
+        // candidate), or problem-major (a candidate)
+        auto deep_first = [&](int tm) {
             Stage st{6, 0, (long long)pl.ents.size(), 0};
+            st.tm = tm;
             int maxp = 0;
             for (const BProb &b : pl.probs) maxp = std::max(maxp, b.nplanes);
             for (int z = maxp - 1; z >= 0; --z)
                 for (int i = 0; i < count; ++i) {
                     const BProb &b = pl.probs[i];
-                    if (!b.S || z >= b.nplanes || b.m == 0 || b.n == 0 || b.k == 0) continue;
-                    for (int x = 0; x < b.mp / 128; ++x)
-                        for (int y = 0; y < b.np / 128; ++y) push(i, x, y, z);
+                    if (b.S <= 0 || z >= b.nplanes || b.m == 0 || b.n == 0 || b.k == 0) continue;
+                    for (int x = 0; x < b.mp / tm; ++x)
+                        for (int y = 0; y < b.np / tm; ++y) push(i, x, y, z);
                 }
             st.count = (long long)pl.ents.size() - st.first;
+            return st;
+        };
+        {
+            Stage st = deep_first(128);
             if (st.count) pl.stages.push_back(st);
-            // the alternative schedule the autotuner tries: problem-major
+            pl.gemm_cands.push_back(deep_first(64));
             Stage alt{6, 0, (long long)pl.ents.size(), 0};
             for (int i = 0; i < count; ++i) {
                 const BProb &b = pl.probs[i];
-                if (!b.S || b.m == 0 || b.n == 0 || b.k == 0) continue;
+                if (b.S <= 0 || b.m == 0 || b.n == 0 || b.k == 0) continue;
                 for (int z = b.nplanes - 1; z >= 0; --z)
                     for (int x = 0; x < b.mp / 128; ++x)
                         for (int y = 0; y < b.np / 128; ++y) push(i, x, y, z);
             }
             alt.count = (long long)pl.ents.size() - alt.first;
-            pl.gemm_alt = alt;
+            pl.gemm_cands.push_back(alt);
         }
         for (int L : Ls) {
             const size_t before = pl.stages.size();
@@ -538,7 +553,12 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 3: hipLaunchKernelGGL(b_finish_cols_kernel, grid, blk, 0, s, P, T); break;
     case 4: MDG_BY_L(b_slice_a_kernel); break;
     case 5: MDG_BY_L(b_slice_b_kernel); break;
-    case 6: hipLaunchKernelGGL(b_diag_i8_kernel, grid, blk, 0, s, P, T); break;
+    case 6:
+        if (st.tm == 64)
+            hipLaunchKernelGGL(b_diag_i8_kernel<64>, grid, blk, 0, s, P, T);
+        else
+            hipLaunchKernelGGL(b_diag_i8_kernel<128>, grid, blk, 0, s, P, T);
+        break;
     case 7: {
         auto rec = [&](auto lc) -> int {
             constexpr int L = decltype(lc)::value;
@@ -567,33 +587,55 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
 static int run_plan(const Plan &pl, hipStream_t s)
 {
     for (const Stage &st : pl.stages) {
-        int rc = launch_stage(pl, (st.kind == 6 && pl.use_alt) ? pl.gemm_alt : st, s);
+        const bool alt = st.kind == 6 && pl.gemm_pick >= 0;
+        int rc = launch_stage(pl, alt ? pl.gemm_cands[pl.gemm_pick] : st, s);
         if (rc) return rc;
     }
     return pl.rb ? residue_batch_run(pl.rb, s) : MDG_SUCCESS;
 }
 
-// Autotune the batch schedule: run the pipeline with each GEMM tile order and
-// keep the faster (both produce identical results).
+// Autotune the batch schedule: run the whole pipeline once with every GEMM
+// stage candidate (tile order and tile size for the slice planes, tile size
+// for the residue planes) and keep the fastest; all produce identical
+// results.
 static int tune_plan(Plan &pl, hipStream_t s)
 {
-    if (pl.gemm_alt.count == 0) return run_plan(pl, s);
-    float ms[2] = {0.f, 0.f};
-    for (int o = 0; o < 2; ++o) {
-        pl.use_alt = (o == 1);
-        hipEvent_t e0, e1;
-        MDG_HIP_CHECK(hipEventCreate(&e0));
-        MDG_HIP_CHECK(hipEventCreate(&e1));
-        MDG_HIP_CHECK(hipEventRecord(e0, s));
-        int rc = run_plan(pl, s);
-        MDG_HIP_CHECK(hipEventRecord(e1, s));
-        MDG_HIP_CHECK(hipEventSynchronize(e1));
-        MDG_HIP_CHECK(hipEventElapsedTime(&ms[o], e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc) return rc;
+    const int nres = pl.rb ? residue_batch_candidates(pl.rb) : 1;
+    int nslice = 0;
+    for (const Stage &st : pl.stages) nslice += st.kind == 6;
+    const int nc = nslice ? 1 + (int)pl.gemm_cands.size() : nres;
+    if (nc <= 1) return run_plan(pl, s);
+    hipEvent_t e0, e1;
+    MDG_HIP_CHECK(hipEventCreate(&e0));
+    MDG_HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int pick = 0, rc = MDG_SUCCESS;
+    for (int c = 0; c < nc && rc == MDG_SUCCESS; ++c) {
+        if (nslice)
+            pl.gemm_pick = c - 1;
+        else
+            residue_batch_select(pl.rb, c);
+        float ms = 0.f;
+        if (hipEventRecord(e0, s) != hipSuccess || (rc = run_plan(pl, s)) || hipEventRecord(e1, s) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            if (!rc) {
+                set_error("timing the batch schedules failed");
+                rc = MDG_ERROR_HIP;
+            }
+            break;
+        }
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
     }
-    pl.use_alt = ms[1] < ms[0];
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (nslice)
+        pl.gemm_pick = pick - 1;
+    else
+        residue_batch_select(pl.rb, pick);
     return MDG_SUCCESS;
 }
 

@@ -425,6 +425,16 @@ __global__ __launch_bounds__(256) void f64_residue_b_kernel(int k, int n, int kp
 // saved buy occupancy, which hides the load latency better (qd n=4096:
 // 1.42 ms against 1.68 ms with four and 1.90 ms with eight)
 constexpr int RES_GB = 1;
+// octo- and hexa-double: the CRT words take most registers (occupancy 1-2
+// waves), so one group of loads at a time leaves every plane load's latency
+// exposed; four groups go out together (hd n=4096: 25.1 ms against 28.9 ms
+// with one and 49.9 ms with eight).  The FP64-modulus planes keep pairs.
+constexpr int RES_GB_WIDE = 4;
+template <int L>
+constexpr int res_gb()
+{
+    return L <= 4 ? RES_GB : RES_GB_WIDE;
+}
 
 template <int NW>
 __device__ __forceinline__ void crt_acc(double (&acc)[NW], double &tq, const ResTab &t, int g, double R)
@@ -462,6 +472,7 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
     const long long *__restrict__ D64 = pl.D64;
     const long long ps = pl.ps;
     const int nck1 = pl.nck1, nck2 = pl.nck2, reduce = pl.reduce;
+    constexpr int GB = res_gb<L>();
     if (idx >= (long long)m * n) return;
     const long long r = idx / n, c = idx - r * n;
     double y[L];
@@ -479,7 +490,10 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
             for (int i0 = 0; i0 < t.nf; i0 += 2 * RES_GB) {
                 long long v[2 * RES_GB];
 #pragma unroll
-                for (int u = 0; u < 2 * RES_GB; ++u) v[u] = (i0 + u < t.nf) ? D64[(long long)(i0 + u) * ps + idx] : 0;
+                for (int u = 0; u < 2 * RES_GB; ++u) {
+                    const long long val = D64[(long long)min(i0 + u, t.nf - 1) * ps + idx];
+                    v[u] = (i0 + u < t.nf) ? val : 0;
+                }
 #pragma unroll
                 for (int u = 0; u < 2 * RES_GB; ++u) {
                     const int i = i0 + u;
@@ -490,20 +504,24 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
                 }
             }
         }
-        for (int g0 = 0; g0 < t.ng8; g0 += RES_GB) {
-            int v[RES_GB][3];
+        for (int g0 = 0; g0 < t.ng8; g0 += GB) {
+            int v[GB][3];
 #pragma unroll
-            for (int u = 0; u < RES_GB; ++u)
+            for (int u = 0; u < GB; ++u)
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
-                    const int i = 3 * (g0 + u) + x;
+                    // unconditional load of a clamped plane, then a select:
+                    // a branch around each load would make each one wait
+                    const int i = 3 * (g0 + u) + x, ic = min(i, t.n8 - 1);
+                    int val;
                     if constexpr (PRE)
-                        v[u][x] = (i < t.n8) ? (int)pl.P8[(long long)i * ps + idx] : 0;
+                        val = (int)pl.P8[(long long)ic * ps + idx];
                     else
-                        v[u][x] = (i < t.n8) ? D8[(long long)i * nck1 * ps + idx] : 0;
+                        val = D8[(long long)ic * nck1 * ps + idx];
+                    v[u][x] = (i < t.n8) ? val : 0;
                 }
 #pragma unroll
-            for (int u = 0; u < RES_GB; ++u) {
+            for (int u = 0; u < GB; ++u) {
                 const int g = g0 + u;
                 if (g >= t.ng8) break;
                 const double *Eg = t.gE + 3 * g;
@@ -524,24 +542,28 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
                 crt_acc<NW>(acc, tq, t, g, mod_pos(V, t.gP[g], t.ginv[g]));
             }
         }
-        for (int j0 = 0; j0 < t.n16; j0 += RES_GB) {
-            int hh[RES_GB], ll[RES_GB], xx[RES_GB];
+        for (int j0 = 0; j0 < t.n16; j0 += GB) {
+            int hh[GB], ll[GB], xx[GB];
 #pragma unroll
-            for (int u = 0; u < RES_GB; ++u) {
+            for (int u = 0; u < GB; ++u) {
                 const bool on = j0 + u < t.n16;
-                const long long j = j0 + u;
+                const long long j = min(j0 + u, t.n16 - 1);
+                int h0, l0, x0;
                 if constexpr (PRE) {
-                    hh[u] = on ? (int)pl.P16[j * ps + idx] : 0;
-                    ll[u] = on ? (int)pl.P16[(t.n16 + j) * ps + idx] : 0;
-                    xx[u] = on ? (int)pl.P16[(2 * t.n16 + j) * ps + idx] : 0;
+                    h0 = (int)pl.P16[j * ps + idx];
+                    l0 = (int)pl.P16[(t.n16 + j) * ps + idx];
+                    x0 = (int)pl.P16[(2 * t.n16 + j) * ps + idx];
                 } else {
-                    hh[u] = on ? Dhh[j * nck1 * ps + idx] : 0;
-                    ll[u] = on ? Dll[j * nck1 * ps + idx] : 0;
-                    xx[u] = on ? Dx[j * nck2 * ps + idx] : 0;
+                    h0 = Dhh[j * nck1 * ps + idx];
+                    l0 = Dll[j * nck1 * ps + idx];
+                    x0 = Dx[j * nck2 * ps + idx];
                 }
+                hh[u] = on ? h0 : 0;
+                ll[u] = on ? l0 : 0;
+                xx[u] = on ? x0 : 0;
             }
 #pragma unroll
-            for (int u = 0; u < RES_GB; ++u) {
+            for (int u = 0; u < GB; ++u) {
                 const int j = j0 + u;
                 if (j >= t.n16) break;
                 long long h = hh[u], l = ll[u], x = xx[u];
@@ -1294,14 +1316,15 @@ __global__ __launch_bounds__(256) void b_residue_b_kernel(const RProb *__restric
     residue_b_at<L, NWU>(e.x, p.k, p.n, p.kp, p.np, p.B, p.ldb, p.F, p.W, t, p.B8, p.B16);
 }
 
-// entry (prob, tile row, tile column, plane): plane z < n8 is narrow modulus
-// z; then wide modulus j = (z - n8) / 3 with v = (z - n8) % 3: 0 hi*hi,
-// 1 lo*lo, 2 (hi, lo) . (lo, hi) of depth 2kp.  The epilogue stores each
-// exact INT32 sum reduced modulo its modulus.
+// entry (prob, tile row, tile column, plane): a TM x TM tile (i8tile.hpp);
+// plane z < n8 is narrow modulus z; then wide modulus j = (z - n8) / 3 with
+// v = (z - n8) % 3: 0 hi*hi, 1 lo*lo, 2 (hi, lo) . (lo, hi) of depth 2kp.
+// The epilogue stores each exact INT32 sum reduced modulo its modulus.
+template <int TM>
 __global__ __launch_bounds__(256) void b_residue_gemm_kernel(const RProb *__restrict__ P, const REnt *__restrict__ T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[I8T_BM * I8T_LDR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[I8T_BM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t As[TM * I8T_LDR];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[TM * I8T_LDR];
     const REnt e = T[blockIdx.x];
     const RProb p = P[e.prob];
     const long long kp = p.kp;
@@ -1323,41 +1346,22 @@ __global__ __launch_bounds__(256) void b_residue_gemm_kernel(const RProb *__rest
         Kd = (int)(v == 2 ? 2 * kp : kp);
         mi = p.n8 + j;
     }
-    i4t acc[4][4];
-    i8_tile_128(Ab + (long long)e.x * I8T_BM * lda, lda, Bb + (long long)e.y * I8T_BM * ldb, ldb, Kd, As, Bs, acc);
+    i4t acc[I8Tile<TM>::IM][I8Tile<TM>::IM];
+    i8_tile<TM>(Ab + (long long)e.x * TM * lda, lda, Bb + (long long)e.y * TM * ldb, ldb, Kd, As, Bs, acc);
     const double md = p.md[mi], inv = p.inv[mi];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const long long row0 = (long long)e.x * I8T_BM + wm * 64, col0 = (long long)e.y * I8T_BM + wn * 64;
+    const long long row0 = (long long)e.x * TM, col0 = (long long)e.y * TM;
+    const int n = p.n;
     if (e.z < p.n8) {
         uint8_t *plane = p.P8 + (long long)e.z * p.ps;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const long long col = col0 + j * 16 + (lane & 15);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const long long row = row0 + i * 16 + (lane >> 4) * 4 + v;
-                    if (row < p.m && col < p.n)
-                        plane[row * p.n + col] = (uint8_t)(int)mod_pos((double)acc[i][j][v], md, inv);
-                }
-            }
+        i8_tile_each<TM>(acc, row0, col0, p.m, n, [&](long long r, long long c, int v) {
+            plane[r * n + c] = (uint8_t)(int)mod_pos((double)v, md, inv);
+        });
     } else {
         const int j = (e.z - p.n8) / 3, v3 = (e.z - p.n8) % 3;
         uint16_t *plane = p.P16 + (long long)(v3 * p.n16 + j) * p.ps;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const long long col = col0 + jj * 16 + (lane & 15);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const long long row = row0 + i * 16 + (lane >> 4) * 4 + v;
-                    if (row < p.m && col < p.n)
-                        plane[row * p.n + col] = (uint16_t)(int)mod_pos((double)acc[i][jj][v], md, inv);
-                }
-            }
+        i8_tile_each<TM>(acc, row0, col0, p.m, n, [&](long long r, long long c, int v) {
+            plane[r * n + c] = (uint16_t)(int)mod_pos((double)v, md, inv);
+        });
     }
 }
 
@@ -1379,11 +1383,14 @@ struct ResBatch {
         int kind;   // 0 residues of A, 1 of B, 2 GEMM tiles, 3 reconstruction
         int L, nwu, nw;
         long long first, count;
+        int tm = 128;   // GEMM tiles: rows and columns
     };
     std::vector<RProb> probs;
     std::vector<REnt> ents;
     std::vector<St> stages;
-    ResTab tab[17] = {};   // per limb count
+    std::vector<St> gemm_cands;   // the GEMM stage with 128- and 64-row tiles
+    int gemm_pick = 0;
+    ResTab tab[17] = {};          // per limb count
     RProb *dprobs = nullptr;
     REnt *dents = nullptr;
     void *arena = nullptr;
@@ -1549,20 +1556,21 @@ int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *co
     // (problem, plane) go to one XCD, so the tiles that re-read a residue
     // row block or column block find it in that XCD's L2.  Planes are dealt
     // to the XCD with the least work so far (deep cross planes first), and
-    // the eight lists are interleaved.
-    {
+    // the eight lists are interleaved.  Built for 128- and 64-row tiles;
+    // the first call times both.
+    auto gemm_stage = [&](int tm) {
         struct Grp {
-            int prob, z, tiles;
+            int prob, z;
             long long work;
         };
         std::vector<Grp> groups;
         for (int i = 0; i < count; ++i) {
             const RProb &b = rb->probs[i];
             if (!b.m) continue;
-            const int tiles = (b.mp / I8T_BM) * (b.np / I8T_BM);
+            const long long tiles = (long long)(b.mp / tm) * (b.np / tm);
             for (int z = 0; z < b.n8 + 3 * b.n16; ++z) {
                 const bool cross = z >= b.n8 && (z - b.n8) % 3 == 2;
-                groups.push_back(Grp{i, z, tiles, (long long)tiles * b.kp * (cross ? 2 : 1)});
+                groups.push_back(Grp{i, z, tiles * b.kp * (cross ? 2 : 1)});
             }
         }
         std::stable_sort(groups.begin(), groups.end(), [](const Grp &a, const Grp &b) { return a.work > b.work; });
@@ -1575,17 +1583,25 @@ int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *co
                 if (load[c] < load[x]) x = c;
             load[x] += g.work;
             const RProb &b = rb->probs[g.prob];
-            for (int tx = 0; tx < b.mp / I8T_BM; ++tx)
-                for (int ty = 0; ty < b.np / I8T_BM; ++ty) lists[x].push_back(REnt{g.prob, tx, ty, g.z});
+            for (int tx = 0; tx < b.mp / tm; ++tx)
+                for (int ty = 0; ty < b.np / tm; ++ty) lists[x].push_back(REnt{g.prob, tx, ty, g.z});
         }
         ResBatch::St st{2, 0, 0, 0, (long long)ents.size(), 0};
+        st.tm = tm;
         size_t longest = 0;
         for (const auto &l : lists) longest = std::max(longest, l.size());
         for (size_t i = 0; i < longest; ++i)
             for (int x = 0; x < NX; ++x)
                 if (i < lists[x].size()) ents.push_back(lists[x][i]);
         st.count = (long long)ents.size() - st.first;
-        if (st.count) rb->stages.push_back(st);
+        return st;
+    };
+    {
+        const ResBatch::St s128 = gemm_stage(128), s64 = gemm_stage(64);
+        if (s128.count) {
+            rb->gemm_cands = {s128, s64};
+            rb->stages.push_back(s128);   // placeholder: run uses gemm_cands[gemm_pick]
+        }
     }
     for (int L : Ls)
         stage(3, L, [&](int i, const RProb &b) {
@@ -1669,7 +1685,13 @@ int residue_batch_run(const ResBatch *rb, hipStream_t s)
         const REnt *T = rb->dents + st.first;
         int rc = MDG_SUCCESS;
         if (st.kind == 2) {
-            hipLaunchKernelGGL(b_residue_gemm_kernel, dim3((unsigned)st.count), dim3(256), 0, s, rb->dprobs, T);
+            const ResBatch::St &g = rb->gemm_cands[rb->gemm_pick];
+            const REnt *TG = rb->dents + g.first;
+            if (g.tm == 64)
+                hipLaunchKernelGGL(b_residue_gemm_kernel<64>, dim3((unsigned)g.count), dim3(256), 0, s, rb->dprobs, TG);
+            else
+                hipLaunchKernelGGL(b_residue_gemm_kernel<128>, dim3((unsigned)g.count), dim3(256), 0, s, rb->dprobs,
+                                   TG);
         } else {
             const ResTab &t = rb->tab[st.L];
             switch (st.L) {
@@ -1683,6 +1705,13 @@ int residue_batch_run(const ResBatch *rb, hipStream_t s)
         MDG_LAUNCH_CHECK("residue batch stage");
     }
     return MDG_SUCCESS;
+}
+
+int residue_batch_candidates(const ResBatch *rb) { return (int)rb->gemm_cands.size(); }
+
+void residue_batch_select(ResBatch *rb, int c)
+{
+    if (c >= 0 && c < (int)rb->gemm_cands.size()) rb->gemm_pick = c;
 }
 
 void residue_batch_free(ResBatch *rb) { delete rb; }
