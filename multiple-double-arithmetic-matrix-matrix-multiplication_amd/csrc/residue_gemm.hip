@@ -169,15 +169,24 @@ __device__ __forceinline__ void entry_residues(const double (&x)[EPT][L], int sh
         *(PT *)o8 = word;
         i = 1;
     }
+    // independent FMA chains per entry: with fewer entries per thread each
+    // chain is split into partial sums (exact integers, so the same v)
+    constexpr int CH = EPT >= 4 ? 1 : (EPT == 2 ? 2 : 4);
+    constexpr int CHW = EPT >= 2 ? 1 : 2;
     for (; i < t.n8; ++i) {
         const double *pw = t.pw37 + i * RES_NC;
         const double md = t.md[i], inv = t.inv[i];
         PT word = 0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-            double v = 0.0;
+            double vp[CH];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) v = fma(ch[e][c], pw[c], v);
+            for (int q = 0; q < CH; ++q) vp[q] = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) vp[c % CH] = fma(ch[e][c], pw[c], vp[c % CH]);
+            double v = vp[0];
+#pragma unroll
+            for (int q = 1; q < CH; ++q) v += vp[q];
             const int r = (int)mod_bal_odd(v * sg[e], md, inv);
             word |= (PT)(((uint32_t)r & 255u) << (8 * e));
         }
@@ -190,11 +199,19 @@ __device__ __forceinline__ void entry_residues(const double (&x)[EPT][L], int sh
         PT hw = 0, lw = 0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-            double vh = 0.0, vl = 0.0;
+            double vhp[CHW], vlp[CHW];
+#pragma unroll
+            for (int q = 0; q < CHW; ++q) vhp[q] = vlp[q] = 0.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                vh = fma(ch[e][c], ph[c], vh);
-                vl = fma(ch[e][c], pl[c], vl);
+                vhp[c % CHW] = fma(ch[e][c], ph[c], vhp[c % CHW]);
+                vlp[c % CHW] = fma(ch[e][c], pl[c], vlp[c % CHW]);
+            }
+            double vh = vhp[0], vl = vlp[0];
+#pragma unroll
+            for (int q = 1; q < CHW; ++q) {
+                vh += vhp[q];
+                vl += vlp[q];
             }
             const double r1 = mod_bal_odd(vh * sg[e], md, inv);                  // |r1| < 2^15
             const int r = (int)mod_bal_odd(fma(r1, 256.0, vl * sg[e]), md, inv);  // < 2^47 in
@@ -357,9 +374,11 @@ __device__ __forceinline__ void f64_entry_residues(const double (&x)[L], int shi
     for (int c = 0; c < NC; ++c) ch[c] = sg * chunk24<NWU>(N, c);
     for (int i = 0; i < t.nf; ++i) {
         const double *pw = t.pw24 + i * RES_NC24;
-        double v = 0.0;
+        // four independent chains (exact integer partial sums)
+        double vp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v = fma(ch[c], pw[c], v);
+        for (int c = 0; c < NC; ++c) vp[c & 3] = fma(ch[c], pw[c], vp[c & 3]);
+        const double v = (vp[0] + vp[1]) + (vp[2] + vp[3]);
         o[i * plane] = (int32_t)mod_bal_odd(v, t.md[i], t.inv[i]);
     }
 }
