@@ -39,6 +39,7 @@ BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 # kernels (C++/TCtest.cu:137-147; the hexa-double figure counted from
 # C++/hexa_double_functions.cu)
 VECTOR_OPS_PER_MAC = {2: 43, 4: 425, 8: 2011, 16: 11800}
+LIMB_NAMES = {2: "double-double", 4: "quad-double", 8: "octo-double", 16: "hexa-double"}
 
 
 def parse():
@@ -48,6 +49,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--L", type=int, default=4)
+    ap.add_argument("--emax", type=int, default=8,
+                    help="leading-limb exponents uniform in [-emax, emax] (config 5: 60)")
     ap.add_argument("--samples", type=int, default=1024, help="entries checked exactly per path")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
@@ -147,8 +150,8 @@ def main():
     m = k = n
     eff_flops = 2.0 * m * n * k
     seeds = (1, 2)
-    A = mdg.fill_random(mdg.GEN_QD, L, m, k, seeds[0], -8, 8)
-    B = mdg.fill_random(mdg.GEN_QD, L, k, n, seeds[1], -8, 8)
+    A = mdg.fill_random(mdg.GEN_QD, L, m, k, seeds[0], -a.emax, a.emax)
+    B = mdg.fill_random(mdg.GEN_QD, L, k, n, seeds[1], -a.emax, a.emax)
     C = mdg.empty(m * n * L)
     torch.cuda.synchronize()
     paths = [p for p in a.paths.split(",") if p]
@@ -305,8 +308,9 @@ def main():
         "vs_baseline": None,
         "dtype": "int8" if head_path == "int8" else ("bf16" if head_path == "bf16" else "f64"),
         "data": "synthetic",
-        "config": {"workload": f"quad-double GEMM C=A*B, m=n=k={n}, 4 fp64 limbs, leading exponent "
-                               f"e in [-8,8], seeds A={seeds[0]} B={seeds[1]}",
+        "config": {"workload": f"{LIMB_NAMES.get(L, str(L) + '-limb')} GEMM C=A*B, m=n=k={n}, {L} fp64 "
+                               f"limbs, leading exponent e in [-{a.emax},{a.emax}], seeds A={seeds[0]} "
+                               f"B={seeds[1]}",
                    "L": L, "m": m, "n": n, "k": k, "path": head_path},
         "paths": results,
         "ratios": ratios,
