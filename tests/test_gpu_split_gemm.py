@@ -199,3 +199,25 @@ def test_int8_and_f64_residues_bit_identical(mdg, orc, torch_cuda, L):
     C64, i64 = mdg.gemm(L, m, n, k, A, B, method="f64_residue")
     assert i8["total_bits"] == i64["total_bits"]
     assert np.array_equal(to_host(C8).view(np.uint64), to_host(C64).view(np.uint64))
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("L", [2, 8])
+def test_split_gemm_leading_dimensions(mdg, orc, torch_cuda, method, L):
+    # operands inside wider row-major buffers (lda > k, ldb > n) and a result
+    # with ldc > n: the same bits as the packed layout, padding untouched
+    m, n, k = 45, 38, 71
+    lda, ldb, ldc = k + 3, n + 5, n + 2
+    A = orc.generate(orc.GEN_QD, L, m, k, 301 + L, -8, 8)
+    B = orc.generate(orc.GEN_QD, L, k, n, 302 + L, -8, 8)
+    Aw = np.full((m, lda, L), 7.0)
+    Aw[:, :k] = A.reshape(m, k, L)
+    Bw = np.full((k, ldb, L), -3.0)
+    Bw[:, :n] = B.reshape(k, n, L)
+    C0, _ = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
+    Cw = to_dev(np.full(m * ldc * L, 5.0))
+    gemm(mdg, L, m, n, k, to_dev(Aw.ravel()), to_dev(Bw.ravel()), method, C=Cw, lda=lda, ldb=ldb, ldc=ldc)
+    torch_cuda.cuda.synchronize()
+    got = to_host(Cw).reshape(m, ldc, L)
+    assert np.array_equal(got[:, :n].view(np.uint64), to_host(C0).reshape(m, n, L).view(np.uint64)), method
+    assert np.all(got[:, n:] == 5.0), "padding columns of C were written"
