@@ -150,3 +150,32 @@ def test_batched_int8_residue_deep_problem_falls_back(mdg, orc, torch_cuda):
     err, _ = orc.normwise_error(L, m, n, k, A, B, loop[-1], nsamp=128)
     assert err <= 4.0 * k * 2.0 ** (-53 * L)
 
+
+
+@pytest.mark.parametrize("method", ["int8", "int8_residue", "vector"])
+def test_batched_padded_leading_dimensions(mdg, orc, torch_cuda, method):
+    # operands and results inside wider buffers: the fused pipeline against
+    # the per-problem calls, bit for bit, padding untouched
+    rng = np.random.default_rng(23)
+    probs, pads = [], []
+    for i in range(10):
+        L = (2, 4)[i % 2]
+        m, n, k = (int(v) for v in rng.integers(16, 150, 3))
+        lda, ldb, ldc = k + 2, n + 3, n + 1
+        A = orc.generate(orc.GEN_QD, L, m, lda, 500 + 2 * i, -8, 8)
+        B = orc.generate(orc.GEN_QD, L, k, ldb, 501 + 2 * i, -8, 8)
+        probs.append({"L": L, "m": m, "n": n, "k": k, "A": to_dev(A), "B": to_dev(B), "lda": lda, "ldb": ldb,
+                      "ldc": ldc, "C": to_dev(np.full(m * ldc * L, 9.0))})
+        pads.append((m, n, ldc, L))
+    mdg.gemm_batched(probs, method=method, persistent=0, spread_bits=16)
+    torch_cuda.cuda.synchronize()
+    loop = [to_host(p["C"]).copy() for p in probs]
+    for (m, n, ldc, L), C in zip(pads, loop):
+        assert np.all(C.reshape(m, ldc, L)[:, n:] == 9.0), "loop wrote padding"
+    for rep in range(2):
+        for p in probs:
+            p["C"].fill_(9.0)
+        mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        for i, (p, C) in enumerate(zip(probs, loop)):
+            assert_bits_equal(to_host(p["C"]), C, f"{method} padded batch rep={rep} problem {i}")
