@@ -36,6 +36,29 @@ inline int diag_words_needed(int S, int w)
 
 constexpr int RECON_BATCH = 16;   // words whose diagonal loads issue together
 
+// out[s] = d[tp - s] for s < K (0 where tp - s < 0), 0 <= tp < NP, tp per
+// lane: the reversed array shifted by NP - 1 - tp in log2(NP) stages of
+// selects (instead of one select per (s, p) pair)
+template <int NP, int K>
+__device__ __forceinline__ void top_pieces(const double (&d)[NP], int tp, double (&out)[K])
+{
+    double r[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) r[i] = d[NP - 1 - i];
+    const int sh = NP - 1 - tp;
+#pragma unroll
+    for (int b = 0; (1 << b) < NP; ++b) {
+        const bool on = (sh >> b) & 1;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double nxt = (i + (1 << b) < NP) ? r[i + (1 << b)] : 0.0;
+            r[i] = on ? nxt : r[i];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) out[s] = (s < NP) ? r[s] : 0.0;
+}
+
 // D(g) returns the INT64 diagonal sum g (0 <= g < S; it is called with every
 // index of the word layout, clamped into range, so that all loads issue as
 // straight-line code); GS = diag_group(w); y receives the L limbs
@@ -106,12 +129,9 @@ __device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, doubl
     for (int l = 0; l < L; ++l) y[l] = 0.0;
     if (tp < 0) return;
     const int sc = sc0 - w * pad;
-    auto piece = [&](int s) {
-        double v = 0.0;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) v = (p == tp - s) ? d[p] : v;
-        return ldexp(v, 2 * ws * (tp - s) + sc);
-    };
+    double top[KMAX];
+    top_pieces<NP, KMAX>(d, tp, top);
+    auto piece = [&](int s) { return ldexp(top[s], 2 * ws * (tp - s) + sc); };
     {
         double f[L + 1];
 #pragma unroll

@@ -39,6 +39,7 @@
 
 #include "i8tile.hpp"
 #include "md_arith.hpp"
+#include "recon.hpp"
 #include "slices.hpp"
 #include "split_common.hpp"
 
@@ -630,13 +631,9 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
         if (tp >= 0) {
             const int sc = er + fc + 4 - 2 * W;
             double piece[K];
+            top_pieces<NP, K>(d, tp, piece);
 #pragma unroll
-            for (int s = 0; s < K; ++s) {
-                double v = 0.0;
-#pragma unroll
-                for (int p = 0; p < NP; ++p) v = (p == tp - s) ? d[p] : v;
-                piece[s] = ldexp(v, 48 * (tp - s) + sc);
-            }
+            for (int s = 0; s < K; ++s) piece[s] = ldexp(piece[s], 48 * (tp - s) + sc);
             fast_renorm<L>(piece, y);
 #pragma unroll
             for (int s = L + 1; s < K; ++s) md_inc_d<L>(y, piece[s]);
