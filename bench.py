@@ -270,6 +270,24 @@ def main():
             orc.gemm_rows(L, r1, r1 + 1, n, k, Ah, Bh, mode=orc.TP_DEKKER, nthreads=1)
             t1 += time.perf_counter() - t0
             r1 += 1
+        # the reference's own functions (oracle/_ref, built from its sources),
+        # one thread, where that library travelled with the tree
+        ref_st = None
+        if orc.ref() is not None:
+            r2, t2, same = 0, 0.0, True
+            while t2 < a.cpu_seconds / 4 and r2 < m:
+                t0 = time.perf_counter()
+                Cr = orc.gemm_rows_ref(L, r2, r2 + 1, n, k, Ah, Bh)
+                t2 += time.perf_counter() - t0
+                same = same and np.array_equal(
+                    Cr.view(np.uint64), orc.gemm_rows(L, r2, r2 + 1, n, k, Ah, Bh, mode=orc.TP_DEKKER,
+                                                      nthreads=nthreads).view(np.uint64))
+                r2 += 1
+            ref_st = {"value": 2.0 * r2 * n * k / t2 / 1e9, "unit": "GFLOP/s", "cores": 1,
+                      "kind": "reference",
+                      "sample": f"{r2} output rows through the reference's Xdf_mul/Xdf_add compiled "
+                                f"from its sources, {t2:.1f} s",
+                      "bit_identical_to_port": bool(same)}
         model = "unknown"
         try:
             for ln in open("/proc/cpuinfo"):
@@ -284,6 +302,7 @@ def main():
                          f"scaled by effective 2*rows*n*k flops",
                "single_thread": {"value": 2.0 * r1 * n * k / t1 / 1e9, "unit": "GFLOP/s", "cores": 1,
                                  "sample": f"{r1} output rows, {t1:.1f} s"},
+               "reference_build": ref_st,
                "cpu_model": model, "host_threads": os.cpu_count()}
         log(f"cpu baseline: {cpu_gflops:.3f} GFLOP/s on {nthreads} threads ({rows} rows); "
             f"{cpu['single_thread']['value']:.4f} GFLOP/s on 1 thread ({r1} rows); {model}")

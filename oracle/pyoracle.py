@@ -223,6 +223,19 @@ def gemm_rows(L, r0, r1, n, k, A, B, mode=TP_DEKKER, nthreads=1):
     return C
 
 
+def gemm_rows_ref(L, r0, r1, n, k, A, B):
+    """Rows [r0, r1) of the host GEMM through the reference-built library
+    (oracle/_ref), or None where it is not built."""
+    R = ref()
+    if R is None:
+        return None
+    A, B = _c(A), _c(B)
+    Asub = np.ascontiguousarray(A[r0 * k * L:r1 * k * L])
+    C = np.zeros((r1 - r0) * n * L)
+    _chk(R.ref_gemm(L, r1 - r0, n, k, _p(Asub), k, _p(B), n, _p(C), n), "ref_gemm rows")
+    return C
+
+
 def squeeze(L, pp, x, use_ref=False):
     x = _c(x)
     nent = x.size // pp
