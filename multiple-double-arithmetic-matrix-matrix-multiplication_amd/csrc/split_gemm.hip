@@ -52,6 +52,7 @@ namespace mdg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
+typedef int i16 __attribute__((ext_vector_type(16)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -295,96 +296,144 @@ __device__ inline void i8_plane_of(int z, int S, int kp, int kmax, int &g, int &
     c = rem;
 }
 
-// Hand-written backend: workgroup tile 256 x 128, 8 waves in 4 x 2, each
-// 64 x 64 = 4 x 4 MFMA 16x16x64 tiles; k-block of 128 bytes staged in LDS
-// (rows padded to 144 bytes: conflict-free 16-byte fragment reads); the
-// next k-block is fetched into registers while the current one is
-// multiplied.  Operand buffers are padded (mp % 256, np % 128, kp % 128 == 0),
-// so the K loop carries no bounds checks.
-__global__ __launch_bounds__(512) void diag_i8_kernel(const int8_t *__restrict__ Aall,
-                                                      const int8_t *__restrict__ Ball, int S, int kp, int kmax,
-                                                      int m, int n, int tiles_m, int tiles_n,
-                                                      int *__restrict__ D)
+// Hand-written backend: workgroup tile 256 x 256, 4 waves in 2 x 2, each
+// 128 x 128 = 4 x 4 tiles of v_mfma_i32_32x32x32_i8 (256 INT32 sums per lane,
+// all in the accumulation registers: one wave per SIMD).  Per k-block of 128
+// bytes a wave reads 256 fragment rows from LDS for 64 MFMAs, so LDS traffic
+// (reads and the block's writes) stays below the MFMA time.  k-blocks are
+// double-buffered in LDS (2 x 2 x 256 rows x 144 bytes = 144 KiB; rows padded
+// by 16 bytes: conflict-free 16-byte fragment reads): one barrier per
+// k-block.  Operand buffers are padded (mp % 256 == np % 256 == kp % 128 ==
+// 0), so the K loop carries no bounds checks.
+__global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__ Aall,
+                                                           const int8_t *__restrict__ Ball, int S, int kp, int kmax,
+                                                           int m, int n, int tiles_m, int tiles_n,
+                                                           int *__restrict__ D)
 {
-    constexpr int BM = 256, BN = 128, BK = 128, LDR = BK + 16;
-    __shared__ __attribute__((aligned(16))) int8_t As[BM * LDR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[BN * LDR];
+    constexpr int BM = 256, BK = 128, LDR = BK + 16, TILE = BM * LDR, FP = BM / 32;
+    __shared__ __attribute__((aligned(16))) int8_t lds[4 * TILE];   // A0 B0 A1 B1
 
     int g, c;
     i8_plane_of((int)blockIdx.z, S, kp, kmax, g, c);
-    int first = 0;
-    for (int h = 0; h < g; ++h) first += i8_chunks(h, kp, kmax);
+    const int first = i8_first_plane(g, kp, kmax);
     const int k0 = c * kmax;
     const int k1 = min((g + 1) * kp, k0 + kmax);
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const long long ld = (long long)S * kp;
     const int8_t *Ab = Aall + (long long)tm * BM * ld + k0;
-    const int8_t *Bb = Ball + (long long)tn * BN * ld + (long long)(S - 1 - g) * kp + k0;
-    const int Kd = k1 - k0;
+    const int8_t *Bb = Ball + (long long)tn * BM * ld + (long long)(S - 1 - g) * kp + k0;
+    const int nkb = (k1 - k0) / BK;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int lrow = lane & 15, lk = (lane >> 4) * 16;
+    const int lrow = lane & 31, lk = (lane >> 5) * 16;
 
-    i4 acc[4][4];
+    i16 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (i4){0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
 
-    // thread tid moves 16 bytes of rows tid/8 + 64 i (A: i < 4, B: i < 2)
+    // thread tid moves 16 bytes of rows tid/8 + 32 f, f < 8, of each operand:
+    // a per-thread 32-bit offset from a uniform base (one address register)
     const int frow = tid >> 3, fseg = (tid & 7) * 16;
-    const long long foff = (long long)frow * ld + fseg, fstep = 64 * ld;
-    const int soff = frow * LDR + fseg, sstep = 64 * LDR;
-    i4 ra0, ra1, ra2, ra3, rb0, rb1;
-#define MDG_FETCH_I8(kb)                                     \
-    do {                                                     \
-        ra0 = *(const i4 *)(Ab + foff + (kb));               \
-        ra1 = *(const i4 *)(Ab + foff + fstep + (kb));       \
-        ra2 = *(const i4 *)(Ab + foff + 2 * fstep + (kb));   \
-        ra3 = *(const i4 *)(Ab + foff + 3 * fstep + (kb));   \
-        rb0 = *(const i4 *)(Bb + foff + (kb));               \
-        rb1 = *(const i4 *)(Bb + foff + fstep + (kb));       \
+    const unsigned foff = (unsigned)(frow * ld + fseg);
+    const long long fstep = 32 * ld;
+    const int soff = frow * LDR + fseg, sstep = 32 * LDR;
+    // two register sets: the global loads run two k-blocks ahead of the
+    // MFMAs (set kb & 1 receives block kb + 2 while set (kb + 1) & 1, holding
+    // block kb + 1, waits to be written to LDS at the end of block kb)
+    i4 ra[2][FP], rb[2][FP];
+#define MDG_WIDE_FETCH(SET, KB)                                                     \
+    do {                                                                            \
+        const long long kofs_ = (long long)(KB) * BK;                               \
+        _Pragma("unroll") for (int f = 0; f < FP; ++f)                              \
+        {                                                                           \
+            ra[SET][f] = *(const i4 *)(Ab + (f * fstep + kofs_) + foff);            \
+            rb[SET][f] = *(const i4 *)(Bb + (f * fstep + kofs_) + foff);            \
+        }                                                                           \
     } while (0)
-    MDG_FETCH_I8(0);
-    for (int kb = 0; kb < Kd; kb += BK) {
-        *(i4 *)&As[soff] = ra0;
-        *(i4 *)&As[soff + sstep] = ra1;
-        *(i4 *)&As[soff + 2 * sstep] = ra2;
-        *(i4 *)&As[soff + 3 * sstep] = ra3;
-        *(i4 *)&Bs[soff] = rb0;
-        *(i4 *)&Bs[soff + sstep] = rb1;
-        __syncthreads();
-        if (kb + BK < Kd) MDG_FETCH_I8(kb + BK);
+    MDG_WIDE_FETCH(0, 0);
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 64) {
-            i4 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = *(const i4 *)&As[(wm * 64 + i * 16 + lrow) * LDR + ks + lk];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                b[j] = *(const i4 *)&Bs[(wn * 64 + j * 16 + lrow) * LDR + ks + lk];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
+    for (int f = 0; f < FP; ++f) {
+        *(i4 *)&lds[soff + f * sstep] = ra[0][f];
+        *(i4 *)&lds[TILE + soff + f * sstep] = rb[0][f];
     }
-#undef MDG_FETCH_I8
-    // acc element v of tile (i, j) is row 4*(lane/16)+v, col lane%16
+    MDG_WIDE_FETCH(1, min(1, nkb - 1));
+    __syncthreads();
+    // fragments: lane l holds row l % 32, bytes 16 (l / 32) .. + 16 of a
+    // 32-byte k-step, for A and B alike
+    const int fra = (wm * 128 + lrow) * LDR + lk, frb = TILE + (wn * 128 + lrow) * LDR + lk;
+    i4 fa[2][4], fb[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) fa[0][i] = *(const i4 *)&lds[fra + i * 32 * LDR];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) fb[0][j] = *(const i4 *)&lds[frb + j * 32 * LDR];
+    // one k-block: SET receives block kb + 2 (the last one again near the
+    // end: no branch, and nothing reads it); set SET ^ 1 goes to LDS
+#define MDG_WIDE_BLOCK(SET)                                                                              \
+    do {                                                                                                 \
+        const int8_t *buf = lds + (kb & 1) * 2 * TILE;                                                   \
+        int8_t *nbuf = lds + ((kb + 1) & 1) * 2 * TILE;                                                  \
+        MDG_WIDE_FETCH(SET, min(kb + 2, nkb - 1));                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                                               \
+        _Pragma("unroll") for (int t = 0; t < BK / 32 - 1; ++t)                                          \
+        {                                                                                                \
+            const int cur = t & 1, nxt = cur ^ 1;                                                        \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i) fa[nxt][i] =                                   \
+                *(const i4 *)&buf[fra + i * 32 * LDR + (t + 1) * 32];                                    \
+            _Pragma("unroll") for (int j = 0; j < 4; ++j) fb[nxt][j] =                                   \
+                *(const i4 *)&buf[frb + j * 32 * LDR + (t + 1) * 32];                                    \
+            __builtin_amdgcn_sched_barrier(0);                                                           \
+            _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)  \
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][i], fb[cur][j], acc[i][j], 0, 0, 0); \
+            __builtin_amdgcn_sched_barrier(0);                                                           \
+        }                                                                                                \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)      \
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][i], fb[1][j], acc[i][j], 0, 0, 0);   \
+        _Pragma("unroll") for (int f = 0; f < FP; ++f)                                                   \
+        {                                                                                                \
+            *(i4 *)&nbuf[soff + f * sstep] = ra[(SET) ^ 1][f];                                           \
+            *(i4 *)&nbuf[TILE + soff + f * sstep] = rb[(SET) ^ 1][f];                                    \
+        }                                                                                                \
+        _Pragma("unroll") for (int q = 0; q < 16; ++q)                                                   \
+        {                                                                                                \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                           \
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);                                           \
+        }                                                                                                \
+        __builtin_amdgcn_sched_barrier(0);                                                               \
+        __syncthreads();                                                                                 \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) fa[0][i] = *(const i4 *)&nbuf[fra + i * 32 * LDR]; \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j) fb[0][j] = *(const i4 *)&nbuf[frb + j * 32 * LDR]; \
+    } while (0)
+    // k-steps 0-2 of a block read the fragments of step t + 1 from LDS while
+    // the 16 MFMAs of step t run; step 3's MFMAs are interleaved with the 16
+    // LDS writes of the next block into the spare buffer (last read before
+    // the previous barrier); step 0 of the next block is read after the
+    // barrier while the last MFMAs drain
+    int kb = 0;
+    for (; kb + 1 < nkb; kb += 2) {
+        MDG_WIDE_BLOCK(0);
+        ++kb;
+        MDG_WIDE_BLOCK(1);
+        --kb;
+    }
+    if (kb < nkb) MDG_WIDE_BLOCK(0);
+#undef MDG_WIDE_BLOCK
+#undef MDG_WIDE_FETCH
+    // acc element v of tile (i, j) is row 8*(v/4) + 4*(lane/32) + v%4, col lane%32
     int *plane = D + (long long)(first + c) * m * n;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const long long col = (long long)tn * BN + wn * 64 + j * 16 + (lane & 15);
+            const long long col = (long long)tn * BM + wn * 128 + j * 32 + (lane & 31);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const long long row = (long long)tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+            for (int v = 0; v < 16; ++v) {
+                const long long row = (long long)tm * BM + wm * 128 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                 if (row < m && col < n) plane[row * n + col] = acc[i][j][v];
             }
         }
@@ -924,24 +973,24 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        const int mp8 = (int)round_up(m, 256);
+        const int mp8 = (int)round_up(m, 256), np8 = (int)round_up(n, 256);
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mp8 * S * kp, 2);
-        int8_t *Ball = (int8_t *)scratch(s, (size_t)np * S * kp, 3);
+        int8_t *Ball = (int8_t *)scratch(s, (size_t)np8 * S * kp, 3);
         int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
         MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
         hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp8 * kp, 256)), dim3(256), 0,
                            s, m, k, mp8, kp, A, lda, E, S, w, Aall);
-        hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np * kp, 256)), dim3(256), 0,
-                           s, k, n, kp, np, B, ldb, F, S, w, Ball);
+        hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np8 * kp, 256)), dim3(256), 0,
+                           s, k, n, kp, np8, B, ldb, F, S, w, Ball);
         tm.mark(2);
         int lib_engine = 1;   // rocBLAS or hipBLASLt, as the library planes ran
         auto run_engine = [&](int engine) -> int {
             if (engine == 1) return i8_planes_library(Aall, Ball, S, kp, kmax, m, n, D, s, &lib_engine);
             MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
-            const int tiles_m8 = mp8 / 256, tiles_n = np / 128;
-            dim3 grid(tiles_m8 * tiles_n, 1, (unsigned)nplanes);
-            hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(512), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
-                               tiles_n, D);
+            const int tiles_m8 = mp8 / 256, tiles_n8 = np8 / 256;
+            dim3 grid(tiles_m8 * tiles_n8, 1, (unsigned)nplanes);
+            hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
+                               tiles_n8, D);
             MDG_LAUNCH_CHECK("diag_i8_kernel");
             return MDG_SUCCESS;
         };
