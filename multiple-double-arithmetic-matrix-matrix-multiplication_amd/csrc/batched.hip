@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict_
     const int8_t *Ab = p.Aall + (long long)e.x * TM * ld + k0;
     const int8_t *Bb = p.Ball + (long long)e.y * TM * ld + (long long)(S - 1 - g) * kp + k0;
     i4t acc[I8Tile<TM>::IM][I8Tile<TM>::IM];
-    i8_tile<TM>(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
+    i8_tile<TM, I8Tile<TM>::RING>(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
     int *plane = p.D + (long long)e.z * p.m * p.n;
     const int n = p.n;
     i8_tile_each<TM>(acc, (long long)e.x * TM, (long long)e.y * TM, p.m, n,

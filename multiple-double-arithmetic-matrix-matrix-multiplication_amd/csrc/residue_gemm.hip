@@ -1363,7 +1363,7 @@ __global__ __launch_bounds__(256) void b_residue_gemm_kernel(const RProb *__rest
         mi = p.n8 + j;
     }
     i4t acc[I8Tile<TM>::IM][I8Tile<TM>::IM];
-    i8_tile<TM>(Ab + (long long)e.x * TM * lda, lda, Bb + (long long)e.y * TM * ldb, ldb, Kd, As, Bs, acc);
+    i8_tile<TM, I8Tile<TM>::RING>(Ab + (long long)e.x * TM * lda, lda, Bb + (long long)e.y * TM * ldb, ldb, Kd, As, Bs, acc);
     const double md = p.md[mi], inv = p.inv[mi];
     const long long row0 = (long long)e.x * TM, col0 = (long long)e.y * TM;
     const int n = p.n;
