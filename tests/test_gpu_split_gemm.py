@@ -146,6 +146,19 @@ def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
         assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
 
 
+def test_int8_engines_agree_multi_tile(mdg, orc, torch_cuda):
+    # the MFMA kernel's 256 x 256 tiles: 3 x 3 tiles with ragged last rows
+    # and columns, and k-blocks per plane both odd and even (kp = 1152: 9 per
+    # diagonal step), against rocBLAS bit for bit
+    L, m, n, k = 2, 600, 530, 1100
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 71, -12, 12))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 72, -12, 12))
+    C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8_rocblas")
+    C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
+    assert i1["backend"] == 1 and i2["backend"] == 2
+    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
+
+
 def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):
     # k > 8192: the FP64 partial sums are moved into INT64 more than once
     L, m, n, k = 4, 40, 36, 20000
