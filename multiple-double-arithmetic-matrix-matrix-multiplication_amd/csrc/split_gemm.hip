@@ -306,9 +306,9 @@ __device__ inline void i8_plane_of(int z, int S, int kp, int kmax, int &g, int &
 // k-block.  Operand buffers are padded (mp % 256 == np % 256 == kp % 128 ==
 // 0), so the K loop carries no bounds checks.
 __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__ Aall,
-                                                           const int8_t *__restrict__ Ball, int S, int kp, int kmax,
-                                                           int m, int n, int tiles_m, int tiles_n,
-                                                           int *__restrict__ D)
+                                                      const int8_t *__restrict__ Ball, int S, int kp, int kmax,
+                                                      int m, int n, int tiles_m, int tiles_n,
+                                                      int *__restrict__ D)
 {
     constexpr int BM = 256, BK = 128, LDR = BK + 16, TILE = BM * LDR, FP = BM / 32;
     __shared__ __attribute__((aligned(16))) int8_t lds[4 * TILE];   // A0 B0 A1 B1
@@ -338,9 +338,9 @@ __global__ __launch_bounds__(256) void diag_i8_kernel(const int8_t *__restrict__
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
 
     // thread tid moves 16 bytes of rows tid/8 + 32 f, f < 8, of each operand:
-    // a per-thread 32-bit offset from a uniform base (one address register)
+    // a per-thread offset from uniform bases
     const int frow = tid >> 3, fseg = (tid & 7) * 16;
-    const unsigned foff = (unsigned)(frow * ld + fseg);
+    const long long foff = frow * ld + fseg;
     const long long fstep = 32 * ld;
     const int soff = frow * LDR + fseg, sstep = 32 * LDR;
     // two register sets: the global loads run two k-blocks ahead of the
