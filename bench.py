@@ -40,6 +40,9 @@ BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 # C++/hexa_double_functions.cu)
 VECTOR_OPS_PER_MAC = {2: 43, 4: 425, 8: 2011, 16: 11800}
 LIMB_NAMES = {2: "double-double", 4: "quad-double", 8: "octo-double", 16: "hexa-double"}
+# INT8 slice-plane engines (mdg_set_int8_backend) and their names in the output
+ENGINE_OF_PATH = {"int8": 0, "int8_tensile": 3, "int8_tile256": 4}
+ENGINE_NAMES = {1: "rocblas", 2: "band_i8_kernel", 3: "hipblaslt", 4: "diag_i8_kernel"}
 
 
 def parse():
@@ -55,12 +58,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--paths", default="int8,int8_residue,f64_residue,int8_mfma,f64,bf16,vector",
-                    help="int8 = INT8 slices with the autotuned diagonal-GEMM engine, int8_mfma = "
-                         "forced to this library's INT8 MFMA kernel, int8_residue = INT8 residues "
-                         "modulo coprime moduli (CRT), f64_residue = the same on FP64 MFMA with 21-bit "
-                         "primes, f64 = FP64-MFMA split, bf16 = BF16 slices, "
-                         "vector = one thread per output entry")
+    ap.add_argument("--paths", default="int8,int8_tensile,int8_tile256,int8_residue,f64_residue,f64,bf16,vector",
+                    help="int8 = INT8 slices on this library's band kernel (the default engine), "
+                         "int8_tensile = the same slices on hipBLASLt's Tensile INT8 GEMMs (comparison "
+                         "only), int8_tile256 = the same on this library's 256 x 256 single-diagonal "
+                         "kernel, int8_residue = INT8 residues modulo coprime moduli (CRT), f64_residue = "
+                         "the same on FP64 MFMA with 21-bit primes, f64 = FP64-MFMA split, bf16 = BF16 "
+                         "slices, vector = one thread per output entry")
     ap.add_argument("--elementwise", action="store_true",
                     help="time the elementwise operations instead of the GEMM")
     ap.add_argument("--ew-entries", type=int, default=1 << 22)
@@ -168,8 +172,8 @@ def main():
             s = mdg.fill_random(mdg.GEN_QD, L, 64, 64, 3, -8, 8)
             mdg.gemm(L, 64, 64, 64, s, s, method="vector")
         log(f"{path}: warmup {warm}, steps {steps}")
-        method = "int8" if path == "int8_mfma" else path
-        mdg.set_int8_backend(2 if path == "int8_mfma" else 0)
+        method = "int8" if path in ENGINE_OF_PATH else path
+        mdg.set_int8_backend(ENGINE_OF_PATH.get(path, 0))
         for _ in range(warm):
             mdg.gemm(L, m, n, k, A, B, method=method, C=C)
         torch.cuda.synchronize()
@@ -195,7 +199,7 @@ def main():
                          "slices": info["slices"], "slice_bits": info["slice_bits"],
                          "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
                          "spread": [info["spread_a"], info["spread_b"]], "steps": steps,
-                         "engine": {1: "rocblas", 2: "mfma", 3: "hipblaslt"}.get(info["backend"], "vector")}
+                         "engine": ENGINE_NAMES.get(info["backend"], "vector") if path != "vector" else "vector"}
         log(f"{path}: {ms:.2f} ms/step, {eff_flops / ms / 1e6:.1f} GFLOP/s effective, "
             f"stages {np.round(stage, 3).tolist()} ms")
     mdg.set_profiling(False)
@@ -219,8 +223,7 @@ def main():
     if head_path == "int8":
         work = P * 2.0 * m * n * k
         peak, unit = INT8_PEAK_TOPS, "TFLOP/s"
-        # the library engines (rocBLAS, hipBLASLt) both run Tensile INT8 kernels
-        kernel = "diag_i8_kernel" if head["engine"] == "mfma" else "int8_plane_gemm"
+        kernel = head["engine"]
     elif head_path == "bf16":
         work = P * 2.0 * m * n * k
         peak, unit = BF16_PEAK_TFLOPS, "TFLOP/s"
@@ -231,12 +234,13 @@ def main():
         kernel = "diag_f64_kernel"
     # launches of the dominant kernel per step: one per INT32 plane (INT8),
     # one diagonal-GEMM launch covering all diagonals otherwise
-    if head_path == "int8":
+    if head_path == "int8" and head["engine"] == "band_i8_kernel":
+        # one launch over the bands of 4 diagonals, one over the narrower last band
+        launches = 1 + (1 if head["slices"] % 4 else 0)
+    elif head_path == "int8" and head["engine"] in ("rocblas", "hipblaslt"):
         kp = -(-k // 128) * 128
         kmax = (131071 // kp) * kp if kp <= 131071 else 130944
         launches = sum(-(-((g + 1) * kp) // kmax) for g in range(head["slices"]))
-        if head["engine"] == "mfma":
-            launches = 1
     else:
         launches = 1
     achieved = work / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None

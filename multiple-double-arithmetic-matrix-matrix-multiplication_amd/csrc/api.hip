@@ -170,7 +170,7 @@ int mdg_release_workspace(void) { return release_scratch(); }
 
 int mdg_set_int8_backend(int backend)
 {
-    MDG_REQUIRE(backend >= 0 && backend <= 3, MDG_ERROR_INVALID_VALUE, "unknown INT8 backend %d", backend);
+    MDG_REQUIRE(backend >= 0 && backend <= 4, MDG_ERROR_INVALID_VALUE, "unknown INT8 backend %d", backend);
     g_int8_backend.store(backend);
     return MDG_SUCCESS;
 }

@@ -1,0 +1,351 @@
+// i8band.hpp -- the INT8 slice-pair GEMM: bands of diagonals per workgroup.
+//
+// Diagonal g of the slice product is D_g = sum_{t+u=g} A_t * B_u (A_t, B_u
+// the INT8 slices of A's rows and B's columns, K-contiguous).  Diagonal g on
+// its own is a GEMM of depth (g+1) k that re-reads slices 0..g of both
+// operands; a workgroup here computes a BAND of G consecutive diagonals
+// g0 .. g0+G-1 of one 128 x 128 output tile at once.  Per k-block it streams
+// A_t for t = 0, 1, ... and keeps a sliding window of the G slices
+// B_{g0-t} .. B_{g0+G-1-t} in LDS: every A block meets G B blocks, and every
+// B block is fetched once per k-block instead of once per diagonal, which
+// halves the operand traffic per MFMA against a plain 256 x 256 tile.
+//
+// Operand layout ("blocked"): 128 rows x 128 k-bytes per 16 KiB block,
+// block (tile, kb, slice) at ((tile * nkb + kb) * S + slice) * 16 KiB, so a
+// workgroup's stream of slices of one k-block is contiguous.  Inside a block
+// row r holds its 128 bytes as eight 16-byte chunks in the order
+// chunk ^ swz(r), swz(r) = (r & 7) ^ ((r >> 3) & 1): a block is copied to LDS
+// as-is by direct global->LDS loads (1 KiB contiguous per wave instruction)
+// and the 16-byte MFMA fragment reads of 16 consecutive rows then hit 16
+// distinct 4-bank groups (and 8 rows, 8 groups): no bank conflicts.
+//
+// Exactness: |d_t d_u| <= 2^14, so an INT32 sum over depth K is exact while
+// K < 2^17; a band's t-range is cut into chunks of Tc slices with
+// Tc * kp <= 131071 and every (diagonal, chunk) is its own INT32 plane
+// (plane numbering of split_gemm.hip: ascending g, then chunk).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace mdg {
+
+constexpr int BND_TM = 128;                 // tile rows = tile columns
+constexpr int BND_BK = 128;                 // k-bytes per block
+constexpr int BND_BLK = BND_TM * BND_BK;    // 16 KiB
+
+// swizzled byte position of (row r, k-byte kk) inside a block
+__host__ __device__ __forceinline__ int bnd_pos(int r, int kk)
+{
+    const int sw = (r & 7) ^ ((r >> 3) & 1);
+    return r * BND_BK + ((((kk >> 4) ^ sw) & 7) << 4) + (kk & 15);
+}
+
+// byte offset of block (tile, kb, slice) of an operand of S slices and nkb
+// k-blocks per slice
+__host__ __device__ __forceinline__ long long bnd_block(int tile, int kb, int t, int S, int nkb)
+{
+    return (((long long)tile * nkb + kb) * S + t) * (long long)BND_BLK;
+}
+
+// One work item of the band kernel: diagonals g0 .. g0+gb-1, slices t of A
+// in [t0, t1), writing chunk c = t0 / Tc of each diagonal's planes.
+struct BandItem {
+    short g0, gb, t0, t1;
+};
+
+constexpr int BND_MAX_ITEMS = 240;
+
+struct BandTable {
+    int count;
+    int pad;
+    BandItem item[BND_MAX_ITEMS];
+};
+
+typedef int bnd_i4 __attribute__((ext_vector_type(4)));
+typedef int bnd_i16 __attribute__((ext_vector_type(16)));
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding: vmcnt[3:0] in bits 3:0,
+// vmcnt[5:4] in bits 15:14, expcnt left at its maximum, lgkmcnt 0): every
+// global->LDS copy but the last N instructions, and every LDS access, done
+template <int N>
+__device__ __forceinline__ void bnd_wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt out of range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0070);
+}
+
+// 16 bytes per lane, global -> LDS; the wave's 64 lanes fill 1 KiB at lds_wave
+__device__ __forceinline__ void bnd_dma16(const int8_t *src, int8_t *lds_wave)
+{
+    __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)lds_wave, 16, 0,
+                                     0);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Give every XCD a contiguous block of tiles and walk that block in
+// groups of GROUP tile-rows, so the tiles resident on one XCD share A rows
+// and B columns in its L2.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    const int T = tiles_m * tiles_n;
+    int lin = bid;
+    if ((T & 7) == 0) lin = (bid & 7) * (T >> 3) + (bid >> 3);
+    constexpr int GROUP = 8;
+    const int per_group = GROUP * tiles_n;
+    const int gidx = lin / per_group;
+    const int first_m = gidx * GROUP;
+    const int gsize = min(tiles_m - first_m, GROUP);
+    const int in = lin - gidx * per_group;
+    tm = first_m + in % gsize;
+    tn = in / gsize;
+}
+
+// first INT32 plane of diagonal g when every diagonal is cut into chunks of
+// Tc slices: sum_{h<g} ceil((h+1)/Tc)
+__host__ __device__ __forceinline__ int bnd_first_plane(int g, int Tc)
+{
+    const int q = g / Tc, r = g - q * Tc;
+    return Tc * q * (q + 1) / 2 + r * (q + 1);
+}
+
+// Band kernel.  Grid: x = output tiles (tiles_m * tiles_n), y = band items.
+// 4 waves in 2 x 2, each 64 x 64 of every diagonal of the band: 2 x 2
+// v_mfma_i32_32x32x32_i8 tiles per diagonal, G * 64 INT32 sums per lane in
+// the accumulation registers.
+//
+// Step s of the workgroup: k-block kb = s / nsk, slice t = t0 + s % nsk.
+// The A stream holds A_t[kb] at position s; the B stream holds, per k-block,
+// B_u for u = g0+gb-1-t0 down to max(0, g0-t1+1), and step s needs its
+// positions s .. s+gb-1 (the window).  Position p of the A stream lives in
+// LDS slot p % RA, of the B stream in slot p % RB; loads run PA steps
+// ahead.  Each step: wait for this wave's loads of step s, barrier (every
+// wave's loads of step s landed, every wave done with step s-1), issue the
+// loads of step s+PA into the slots step s-1 freed, multiply.  A chunk that
+// ends before the band's last diagonal (t1 <= g0) pads each k-block with
+// gb-1 steps that only load B.
+template <int G, int PA>
+__global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
+                                                      int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
+                                                      BandTable tab, int *__restrict__ D)
+{
+    constexpr int RA = PA + 1, RB = PA + G;
+    // slots: RA of the A stream, RB of the B stream, one block of zeros
+    __shared__ __attribute__((aligned(1024))) int8_t lds[(RA + RB + 1) * BND_BLK];
+    static_assert((RA + RB + 1) * BND_BLK <= 163840, "LDS ring exceeds 160 KiB");
+
+    const BandItem it = tab.item[blockIdx.y];
+    const int g0 = it.g0, t0 = it.t0, t1 = it.t1;
+    constexpr int gb = G;   // items of this launch have exactly G diagonals
+    const int nA = t1 - t0;
+    const int nsk = (t1 <= g0 + 1) ? nA + gb - 1 : g0 + gb - t0;
+    const int uhi = g0 + gb - 1 - t0;
+    const int nsteps = nkb * nsk;
+
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave >> 1, wn = wave & 1;
+    const long long tstride = (long long)nkb * S * BND_BLK;
+    const int8_t *Arow = Ab + tm * tstride + wave * 4096 + lane * 16;
+    const int8_t *Brow = Bb + tn * tstride + wave * 4096 + lane * 16;
+    int8_t *ldsA = lds + wave * 4096;
+    int8_t *ldsB = lds + RA * BND_BLK + wave * 4096;
+
+    // issue cursors: A position pa = s + PA, B position pb = s + PA + gb - 1,
+    // as (k-block, index within the k-block) and ring slot
+    int a_kb = 0, a_i = 0, a_slot = 0, b_kb = 0, b_i = 0, b_slot = 0;
+    auto issue_a = [&]() {
+        int kb = a_kb, i = a_i;
+        if (kb >= nkb) {   // past the end: reload the last block (in bounds, nobody reads it)
+            kb = nkb - 1;
+            i = nsk - 1;
+        }
+        const int t = t0 + min(i, nA - 1);
+        const int8_t *src = Arow + ((long long)kb * S + t) * BND_BLK;
+        int8_t *dst = ldsA + a_slot * BND_BLK;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bnd_dma16(src + q * 1024, dst + q * 1024);
+        if (++a_i == nsk) {
+            a_i = 0;
+            ++a_kb;
+        }
+        if (++a_slot == RA) a_slot = 0;
+    };
+    auto issue_b = [&]() {
+        int kb = b_kb, i = b_i;
+        if (kb >= nkb) {
+            kb = nkb - 1;
+            i = nsk - 1;
+        }
+        const int8_t *src = Brow + ((long long)kb * S + (uhi - i)) * BND_BLK;
+        int8_t *dst = ldsB + b_slot * BND_BLK;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bnd_dma16(src + q * 1024, dst + q * 1024);
+        if (++b_i == nsk) {
+            b_i = 0;
+            ++b_kb;
+        }
+        if (++b_slot == RB) b_slot = 0;
+    };
+
+    // prologue: B positions 0 .. gb-2, then the pairs (A p, B p+gb-1), p < PA
+    for (int p = 0; p < gb - 1; ++p) issue_b();
+#pragma unroll
+    for (int p = 0; p < PA; ++p) {
+        issue_a();
+        issue_b();
+    }
+
+    // fragment addresses: lane l reads row (or column) l % 32 of a 32-row
+    // group and k-bytes 16 (l / 32) .. +16 of each 32-byte k-step
+    const int lrow = lane & 31, hk = lane >> 5;
+    const int sw = (lrow & 7) ^ ((lrow >> 3) & 1);
+    int koff[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) koff[ks] = (((2 * ks + hk) ^ sw) & 7) << 4;
+    const int8_t *fragA = lds + (wm * 64 + lrow) * BND_BK;
+    const int8_t *fragB = lds + RA * BND_BLK + (wn * 64 + lrow) * BND_BK;
+    {
+        // the zero block (slot RB of the B ring); the first barrier of the
+        // main loop orders these writes before every read
+        bnd_i4 *z = (bnd_i4 *)(lds + (RA + RB) * BND_BLK);
+#pragma unroll
+        for (int i = 0; i < BND_BLK / (16 * 256); ++i) z[threadIdx.x + 256 * i] = (bnd_i4){0, 0, 0, 0};
+    }
+
+    bnd_i16 acc[G][2][2];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[j][i][jj][v] = 0;
+
+    // step state: q = index of step s within its k-block, sa = its A slot,
+    // sb = the B slot of its window position s + gb - 1 (diagonal j = 0);
+    // pa / pb[j] = fragment bases of the step, the zero block for diagonals
+    // j < jmin (slice u = g0 + j - t < 0: the last G-1 steps of a k-block) and
+    // for every diagonal of a padding step (q >= nA): no branch near the MFMAs
+    int q = 0, sa = 0, sb = gb - 1;
+    const int8_t *pa = fragA;
+    const int8_t *pb[G];
+    auto set_step = [&]() {
+        const int t = t0 + q;
+        const int jmin = (q < nA) ? max(0, t - g0) : G;
+        pa = fragA + sa * BND_BLK;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            // slot (sb - j) mod RB, or RB (the zero block) when j < jmin;
+            // integer masks instead of selects keep the step free of branches
+            int sj = sb - j;
+            sj += (sj >> 31) & RB;
+            sj += (RB - sj) & -(int)(j < jmin);
+            pb[j] = fragB + sj * BND_BLK;
+        }
+    };
+    bnd_i4 a[2][2], b[2][G][2];
+    auto fetch = [&](int buf, int ks) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[buf][i] = *(const bnd_i4 *)(pa + i * 4096 + koff[ks]);
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) b[buf][j][jj] = *(const bnd_i4 *)(pb[j] + jj * 4096 + koff[ks]);
+    };
+    auto mfma = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj)
+                    acc[j][i][jj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[buf][i], b[buf][j][jj], acc[j][i][jj], 0, 0, 0);
+    };
+
+    // The barrier of step s sits inside its last k-step: wait for this
+    // wave's loads of step s+1 (and for its own fragment reads), barrier,
+    // refill the slots step s released with the loads of step s+1+PA, read
+    // step s+1's first fragments, and only then run step s's last MFMAs,
+    // which hide the latency of those reads.  The ring position p of a load
+    // issued at barrier s is p = s+1+PA: slot p % RA = s % RA (step s's A
+    // block); the B position s+PA+gb lands in slot s % RB (the first position
+    // of step s's window, which step s+1 no longer reads).
+    bnd_wait_vm<8 * (PA - 1)>();
+    __builtin_amdgcn_s_barrier();
+    issue_a();
+    issue_b();
+    set_step();
+    fetch(0, 0);
+    // the scheduler places one fragment read (or one global->LDS copy)
+    // after each MFMA, so no burst of reads or copies leaves the MFMA pipe
+    // idle: nr = fragment reads per k-step, 4 * G * ... MFMAs per k-step
+    constexpr int NR = 2 + 2 * G, NM = 4 * G;
+    for (int s = 0; s < nsteps; ++s) {
+#pragma unroll
+        for (int ks = 0; ks < 3; ++ks) {
+            fetch((ks & 1) ^ 1, ks + 1);
+            mfma(ks & 1);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, NM - NR, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        bnd_wait_vm<8 * (PA - 1)>();
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        if (++q == nsk) q = 0;
+        if (++sa == RA) sa = 0;
+        if (++sb == RB) sb = 0;
+        set_step();
+        fetch(0, 0);
+        issue_a();
+        issue_b();
+        mfma(1);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global->LDS copy
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // no global->LDS copy may still be in flight when the workgroup ends
+    bnd_wait_vm<0>();
+
+    // acc element v of a 32 x 32 tile: row 8 (v / 4) + 4 (lane / 32) + v % 4,
+    // column lane % 32
+    const int c = t0 / Tc;
+    const long long mn = (long long)m * n;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const int g = g0 + j;
+        if (t0 > g) continue;   // this chunk adds nothing to diagonal g
+        int *plane = D + (long long)(bnd_first_plane(g, Tc) + c) * mn;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                const long long col = (long long)tn * BND_TM + wn * 64 + jj * 32 + (lane & 31);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const long long row =
+                        (long long)tm * BND_TM + wm * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    if (row < m && col < n) plane[row * n + col] = acc[j][i][jj][v];
+                }
+            }
+    }
+}
+
+}  // namespace mdg

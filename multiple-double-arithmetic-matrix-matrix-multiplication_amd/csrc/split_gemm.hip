@@ -42,7 +42,9 @@
 #include <mutex>
 #include <tuple>
 #include <type_traits>
+#include <vector>
 
+#include "i8band.hpp"
 #include "md_arith.hpp"
 #include "recon.hpp"
 #include "slices.hpp"
@@ -243,24 +245,32 @@ __global__ __launch_bounds__(256) void slice_b_kernel(int k, int n, int kp, int 
     fixed_digits<NW>(N, S, w, [&](int u, int d) { o[(long long)(S - 1 - u) * kp] = digit_as<T>(d); });
 }
 
-// ------------------------------------------------------ tile scheduling --
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Give every XCD a contiguous block of tiles and walk that block in
-// groups of GROUP tile-rows, so the tiles resident on one XCD share A rows
-// and B columns in its L2.
-__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
+// INT8 slices in the blocked layout of the band kernel (i8band.hpp): thread
+// per entry (r, kk) of the padded rp x kp domain, kk fastest; slice t of the
+// entry lands in block (r / 128, kk / 128, t) at bnd_pos(r % 128, kk % 128).
+// TRANS: the operand is B (k x n, row stride ldx) and "row" r is column r of B.
+template <int L, bool TRANS>
+__global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int rp, int kp, const double *__restrict__ X,
+                                                        long long ldx, const int *__restrict__ E, int S, int w,
+                                                        int8_t *__restrict__ out)
 {
-    const int T = tiles_m * tiles_n;
-    int lin = bid;
-    if ((T & 7) == 0) lin = (bid & 7) * (T >> 3) + (bid >> 3);
-    constexpr int GROUP = 8;
-    const int per_group = GROUP * tiles_n;
-    const int gidx = lin / per_group;
-    const int first_m = gidx * GROUP;
-    const int gsize = min(tiles_m - first_m, GROUP);
-    const int in = lin - gidx * per_group;
-    tm = first_m + in % gsize;
-    tn = in / gsize;
+    constexpr int NW = FixedWords<L>::NW;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)rp * kp) return;
+    const int r = (int)(idx / kp);
+    const int kk = (int)(idx - (long long)r * kp);
+    int8_t *o = out + bnd_block(r >> 7, kk >> 7, 0, S, kp >> 7) + bnd_pos(r & 127, kk & 127);
+    if (r >= rows || kk >= k || E[r] == ZERO_SCALE) {
+        for (int t = 0; t < S; ++t) o[(long long)t * BND_BLK] = 0;
+        return;
+    }
+    double x[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+        x[l] = TRANS ? X[((long long)kk * ldx + r) * L + l] : X[((long long)r * ldx + kk) * L + l];
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, S * w - 2 - E[r], N);
+    fixed_digits<NW>(N, S, w, [&](int t, int d) { o[(long long)t * BND_BLK] = (int8_t)d; });
 }
 
 // --------------------------------------------------- INT8 diagonal GEMM --
@@ -764,39 +774,58 @@ BlasHandle *blas_handle()
     return b;
 }
 
-static int int8_backend_choice()
+// INT8 slice-plane engines (mdg_set_int8_backend): 0 and 2 -- the band
+// kernel (i8band.hpp), or the 256 x 256 single-diagonal kernel where the
+// band kernel cannot run (one slice pair deeper than an INT32 plane allows,
+// k > 131071); 4 -- the single-diagonal kernel; 1 rocBLAS and 3 hipBLASLt --
+// the vendor libraries, kept only as comparison engines.
+enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
+
+// Band items: bands of G diagonals, each band's t-range cut into chunks of
+// Tc slices, heaviest first (work = slice-pair blocks, padding steps
+// included).  Bands of exactly G diagonals go to `full`; the last band, when
+// S is not a multiple of G, to `part` (its own kernel instance, *gpart wide).
+static bool build_band_tables(int S, int G, int Tc, BandTable &full, BandTable &part, int *gpart)
 {
-    // 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel, 3 hipBLASLt
-    return int8_backend();
+    struct W {
+        BandItem it;
+        long long work;
+    };
+    std::vector<W> v[2];
+    *gpart = S % G;
+    for (int g0 = 0; g0 < S; g0 += G) {
+        const int gb = std::min(G, S - g0);
+        for (int t0 = 0; t0 < g0 + gb; t0 += Tc) {
+            const int t1 = std::min(t0 + Tc, g0 + gb);
+            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + gb - 1 : g0 + gb - t0;
+            v[gb == G ? 0 : 1].push_back(W{BandItem{(short)g0, (short)gb, (short)t0, (short)t1}, steps * gb});
+        }
+    }
+    BandTable *tabs[2] = {&full, &part};
+    for (int c = 0; c < 2; ++c) {
+        if ((int)v[c].size() > BND_MAX_ITEMS) return false;
+        std::stable_sort(v[c].begin(), v[c].end(), [](const W &x, const W &y) { return x.work > y.work; });
+        tabs[c]->count = (int)v[c].size();
+        for (int i = 0; i < tabs[c]->count; ++i) tabs[c]->item[i] = v[c][i].it;
+    }
+    return true;
 }
 
-// Autotuner cache of the INT8 engine per shape bucket: limb count, slice
-// count and the power-of-two buckets of m, n and the padded depth.
-struct TuneKey {
-    int L, S, m, n, kp;
-};
-static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;
-
-static std::tuple<int, int, int, int, int, int> bucket(const TuneKey &k)
+// launch the band kernel over one table (G = gb of its items)
+static int launch_band(int gb, const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
+                       int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
 {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    auto lg = [](int v) { return v <= 1 ? 0 : 32 - __builtin_clz((unsigned)(v - 1)); };
-    return {dev, k.L, k.S, lg(k.m), lg(k.n), lg(k.kp)};
-}
-
-static int tuned_engine(const TuneKey &k)
-{
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(bucket(k));
-    return it == g_tune.end() ? 0 : it->second;
-}
-
-static void remember_engine(const TuneKey &k, int engine)
-{
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[bucket(k)] = engine;
+    if (tab.count == 0) return MDG_SUCCESS;
+    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
+    switch (gb) {
+    case 4: hipLaunchKernelGGL((band_i8_kernel<4, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 3: hipLaunchKernelGGL((band_i8_kernel<3, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 2: hipLaunchKernelGGL((band_i8_kernel<2, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 1: hipLaunchKernelGGL((band_i8_kernel<1, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    default: set_error("band width %d", gb); return MDG_ERROR_INVALID_VALUE;
+    }
+    MDG_LAUNCH_CHECK("band_i8_kernel");
+    return MDG_SUCCESS;
 }
 
 // every K-chunk plane as one INT8 GEMM on the library engines (rocBLAS or
@@ -830,7 +859,7 @@ struct Plan {
     bool wide;        // INT8: widened accumulation
     int flush_blocks; // FP64 / INT8
     int spread_a, spread_b;
-    int backend;      // INT8: 1 library engine (rocBLAS / hipBLASLt), 2 hand-written MFMA kernel
+    int backend;      // INT8: I8E_BAND, I8E_TILE256 or I8E_LIB
 };
 
 // Choose slice width and count for a precision target and the measured
@@ -973,66 +1002,54 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        const int mp8 = (int)round_up(m, 256), np8 = (int)round_up(n, 256);
-        int8_t *Aall = (int8_t *)scratch(s, (size_t)mp8 * S * kp, 2);
-        int8_t *Ball = (int8_t *)scratch(s, (size_t)np8 * S * kp, 3);
+        const int forced = int8_backend();
+        int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
+        constexpr int BAND_G = 4;
+        BandTable tab_full, tab_part;
+        int gpart = 0;
+        if (engine == I8E_BAND && (kp > 131071 || !build_band_tables(S, BAND_G, kmax / kp, tab_full, tab_part, &gpart)))
+            engine = I8E_TILE256;
         int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
-        MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-        hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp8 * kp, 256)), dim3(256), 0,
-                           s, m, k, mp8, kp, A, lda, E, S, w, Aall);
-        hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np8 * kp, 256)), dim3(256), 0,
-                           s, k, n, kp, np8, B, ldb, F, S, w, Ball);
-        tm.mark(2);
-        int lib_engine = 1;   // rocBLAS or hipBLASLt, as the library planes ran
-        auto run_engine = [&](int engine) -> int {
-            if (engine == 1) return i8_planes_library(Aall, Ball, S, kp, kmax, m, n, D, s, &lib_engine);
-            MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
-            const int tiles_m8 = mp8 / 256, tiles_n8 = np8 / 256;
-            dim3 grid(tiles_m8 * tiles_n8, 1, (unsigned)nplanes);
-            hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
-                               tiles_n8, D);
-            MDG_LAUNCH_CHECK("diag_i8_kernel");
-            return MDG_SUCCESS;
-        };
-        const int be = int8_backend_choice();
-        bool planes_done = false;
-        if (be == 1 || be == 2 || be == 3) {
-            p.backend = (be == 2) ? 2 : 1;   // forced library engine: i8_gemm_tn honours it
+        MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+        if (engine == I8E_BAND) {
+            // blocked slices, 128-row tiles
+            const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
+            int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp, 2);
+            int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp, 3);
+            MDG_REQUIRE(Aall && Ball, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+            hipLaunchKernelGGL((slice_blk_kernel<L, false>), dim3(blocks_for<int>((long long)mpb * kp, 256)),
+                               dim3(256), 0, s, m, k, mpb, kp, A, lda, E, S, w, Aall);
+            hipLaunchKernelGGL((slice_blk_kernel<L, true>), dim3(blocks_for<int>((long long)npb * kp, 256)),
+                               dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
+            tm.mark(2);
+            const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
+            rc = launch_band(BAND_G, tab_full, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+            if (!rc) rc = launch_band(gpart, tab_part, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+            if (rc) return rc;
         } else {
-            // autotuned: the first call of a shape bucket times both engines on
-            // the same slices (they produce identical planes) and keeps the faster
-            const TuneKey key{L, S, m, n, kp};
-            p.backend = tuned_engine(key);
-            if (p.backend == 0) {
-                // two rounds, minimum per engine: the first call of an
-                // engine pays one-time costs (code object loading)
-                float ms[3] = {0.f, 1e30f, 1e30f};
-                for (int round = 0; round < 2; ++round)
-                for (int engine = 1; engine <= 2; ++engine) {
-                    float t = 0.f;
-                    hipEvent_t e0, e1;
-                    MDG_HIP_CHECK(hipEventCreate(&e0));
-                    MDG_HIP_CHECK(hipEventCreate(&e1));
-                    MDG_HIP_CHECK(hipEventRecord(e0, s));
-                    rc = run_engine(engine);
-                    MDG_HIP_CHECK(hipEventRecord(e1, s));
-                    MDG_HIP_CHECK(hipEventSynchronize(e1));
-                    MDG_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-                    ms[engine] = std::min(ms[engine], t);
-                    (void)hipEventDestroy(e0);
-                    (void)hipEventDestroy(e1);
-                    if (rc) return rc;
-                }
-                p.backend = (ms[2] < ms[1]) ? 2 : 1;
-                remember_engine(key, p.backend);
-                planes_done = true;   // both engines wrote the same exact planes
+            // row-major K-contiguous slices: A_all[r][t kp + kk], Bt_all[c][(S-1-u) kp + kk]
+            const int mp8 = (int)round_up(m, 256), np8 = (int)round_up(n, 256);
+            int8_t *Aall = (int8_t *)scratch(s, (size_t)mp8 * S * kp, 2);
+            int8_t *Ball = (int8_t *)scratch(s, (size_t)np8 * S * kp, 3);
+            MDG_REQUIRE(Aall && Ball, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+            hipLaunchKernelGGL((slice_a_kernel<L, int8_t>), dim3(blocks_for<int>((long long)mp8 * kp, 256)),
+                               dim3(256), 0, s, m, k, mp8, kp, A, lda, E, S, w, Aall);
+            hipLaunchKernelGGL((slice_b_kernel<L, int8_t>), dim3(blocks_for<int>((long long)np8 * kp, 256)),
+                               dim3(256), 0, s, k, n, kp, np8, B, ldb, F, S, w, Ball);
+            tm.mark(2);
+            if (engine == I8E_LIB) {
+                rc = i8_planes_library(Aall, Ball, S, kp, kmax, m, n, D, s, &lib_engine_used);
+                if (rc) return rc;
+            } else {
+                MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
+                const int tiles_m8 = mp8 / 256, tiles_n8 = np8 / 256;
+                dim3 grid(tiles_m8 * tiles_n8, 1, (unsigned)nplanes);
+                hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
+                                   tiles_n8, D);
+                MDG_LAUNCH_CHECK("diag_i8_kernel");
             }
         }
-        if (!planes_done) {
-            rc = run_engine(p.backend);
-            if (rc) return rc;
-        }
-        lib_engine_used = lib_engine;
+        p.backend = engine;
         tm.mark(3);
         rc = with_diag_words<L>(S, w, [&](auto nw) {
             constexpr int NW = decltype(nw)::value;
@@ -1110,7 +1127,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         info->spread_a = p.spread_a;
         info->spread_b = p.spread_b;
         info->flushed = p.wide ? 1 : 0;
-        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? (p.backend == 1 ? lib_engine_used : 2) : 2;
+        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? (p.backend == I8E_LIB ? lib_engine_used : p.backend) : 2;
         for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;

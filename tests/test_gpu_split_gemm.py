@@ -9,14 +9,15 @@ from conftest import to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_rocblas", "int8_mfma", "int8_hipblaslt", "bf16", "int8_residue", "f64_residue"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_tile256", "int8_hipblaslt", "bf16", "int8_residue", "f64_residue"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
-    """int8 runs the autotuned engine, int8_rocblas rocBLAS, int8_mfma this
-    library's INT8 MFMA kernel and int8_hipblaslt hipBLASLt; all give the
-    same exact diagonal sums."""
-    engine = {"int8_rocblas": 1, "int8_mfma": 2, "int8_hipblaslt": 3}.get(method)
+    """int8 runs the default engine (the hand-written band kernel), int8_mfma
+    forces it, int8_tile256 forces the hand-written single-diagonal 256 x 256
+    kernel, int8_rocblas / int8_hipblaslt the vendor libraries (comparison
+    engines); all give the same exact diagonal sums."""
+    engine = {"int8_rocblas": 1, "int8_mfma": 2, "int8_hipblaslt": 3, "int8_tile256": 4}.get(method)
     if engine:
         mdg.set_int8_backend(engine)
         try:
@@ -112,7 +113,7 @@ def test_split_gemm_rejects_non_finite(mdg, torch_cuda):
     assert ei.value.status == 5
 
 
-@pytest.mark.parametrize("method", ["int8_rocblas", "int8_mfma"])
+@pytest.mark.parametrize("method", ["int8_rocblas", "int8_mfma", "int8_tile256"])
 def test_int8_chunked_diagonals(mdg, orc, torch_cuda, method):
     # S * kp > 2^17: the deepest diagonals are cut into K-chunks whose INT32
     # planes are added in INT64 by the reconstruction
@@ -127,36 +128,73 @@ def test_int8_chunked_diagonals(mdg, orc, torch_cuda, method):
 
 @pytest.mark.parametrize("L", [2, 4])
 def test_int8_engines_agree_bitwise(mdg, orc, torch_cuda, L):
-    # rocBLAS and the MFMA kernel compute the same exact diagonal sums, so the
-    # reconstructed results must be identical bit for bit
+    # every engine computes the same exact diagonal sums, so the reconstructed
+    # results must be identical bit for bit; the default engine is the band
+    # kernel (backend 2), with no library in the default dispatch
     m, n, k = 300, 260, 333
     A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 17, -20, 20))
     B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 18, -20, 20))
     C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8_rocblas")
     C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
     C4, i4 = gemm(mdg, L, m, n, k, A, B, "int8_hipblaslt")
-    assert i1["backend"] == 1 and i2["backend"] == 2 and i4["backend"] == 3
-    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
-    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C4).view(np.uint64))
-    # the autotuner times both engines on the first call of a shape bucket,
-    # then keeps its choice; every call gives the same bits
+    C5, i5 = gemm(mdg, L, m, n, k, A, B, "int8_tile256")
+    assert i1["backend"] == 1 and i2["backend"] == 2 and i4["backend"] == 3 and i5["backend"] == 4
+    for C in (C2, C4, C5):
+        assert np.array_equal(to_host(C1).view(np.uint64), to_host(C).view(np.uint64))
     for _ in range(2):
         C3, i3 = gemm(mdg, L, m, n, k, A, B, "int8")
-        assert i3["backend"] in (1, 2, 3)
+        assert i3["backend"] == 2
         assert np.array_equal(to_host(C3).view(np.uint64), to_host(C1).view(np.uint64))
 
 
 def test_int8_engines_agree_multi_tile(mdg, orc, torch_cuda):
-    # the MFMA kernel's 256 x 256 tiles: 3 x 3 tiles with ragged last rows
-    # and columns, and k-blocks per plane both odd and even (kp = 1152: 9 per
-    # diagonal step), against rocBLAS bit for bit
+    # 3 x 3 tiles of 256 (5 x 5 of the band kernel's 128) with ragged last
+    # rows and columns, and kp = 1152 (9 k-blocks per slice), bit for bit
+    # across the hand-written kernels and rocBLAS
     L, m, n, k = 2, 600, 530, 1100
     A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 71, -12, 12))
     B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 72, -12, 12))
     C1, i1 = gemm(mdg, L, m, n, k, A, B, "int8_rocblas")
     C2, i2 = gemm(mdg, L, m, n, k, A, B, "int8_mfma")
-    assert i1["backend"] == 1 and i2["backend"] == 2
+    C3, i3 = gemm(mdg, L, m, n, k, A, B, "int8_tile256")
+    assert i1["backend"] == 1 and i2["backend"] == 2 and i3["backend"] == 4
     assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
+    assert np.array_equal(to_host(C1).view(np.uint64), to_host(C3).view(np.uint64))
+
+
+@pytest.mark.parametrize("L,m,n,k,emax", [
+    (2, 1, 1, 1, 0),          # S = 5 slices: one band of 4 and a band of 1
+    (4, 130, 3, 257, 8),      # band of 4 x 7 and a remainder band; one-row tiles
+    (8, 129, 257, 128, 30),   # S not a multiple of 4, ragged tiles
+    (16, 64, 72, 300, 60),    # ~110 slices: 28 bands
+    (4, 24, 40, 40000, 4),    # kp = 40064: chunks of 3 slices, padding steps
+    (2, 20, 17, 70000, 2),    # chunks of one slice
+])
+def test_int8_band_kernel_shapes(mdg, orc, torch_cuda, L, m, n, k, emax):
+    # the band kernel (default) against the 256 x 256 single-diagonal kernel
+    # bit for bit, and against the exact checker
+    print(f"band L={L} {m}x{n}x{k}", flush=True)
+    A = orc.generate(orc.GEN_QD, L, m, k, 31 + m, -emax, emax)
+    B = orc.generate(orc.GEN_QD, L, k, n, 32 + n, -emax, emax)
+    dA, dB = to_dev(A), to_dev(B)
+    C2, i2 = gemm(mdg, L, m, n, k, dA, dB, "int8")
+    C4, i4 = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256")
+    assert i2["backend"] == 2 and i4["backend"] == 4, (i2, i4)
+    assert np.array_equal(to_host(C2).view(np.uint64), to_host(C4).view(np.uint64))
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C2), nsamp=256)
+    assert err <= _bound(L, k), (err, i2)
+
+
+def test_int8_deep_k_falls_back_to_tile256(mdg, orc, torch_cuda):
+    # one slice pair deeper than an INT32 plane allows (kp > 131071): the
+    # band kernel cannot cut its t-range finer, the 256 x 256 kernel runs
+    L, m, n, k = 2, 8, 12, 131100
+    A = orc.generate(orc.GEN_QD, L, m, k, 41, -2, 2)
+    B = orc.generate(orc.GEN_QD, L, k, n, 42, -2, 2)
+    C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), "int8")
+    assert info["backend"] == 4, info
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=96)
+    assert err <= _bound(L, k), (err, info)
 
 
 def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):
