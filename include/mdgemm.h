@@ -21,6 +21,8 @@
 #ifndef MDGEMM_H
 #define MDGEMM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -143,6 +145,28 @@ int mdg_vector_gemm(int L, int m, int n, int k, const double *A, long long lda,
 int mdg_gemm(int L, int m, int n, int k, const double *A, long long lda,
              const double *B, long long ldb, double *C, long long ldc,
              int method, int target_bits, mdg_gemm_info *info, void *stream);
+
+/* ----------------------------------------------------------------------
+ * mdg_gemm without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results (bit for bit) and errors as mdg_gemm, for the methods
+ * VECTOR, F64_SPLIT, INT8_OZAKI (always the hand-written INT8 kernels) and
+ * BF16_OZAKI; the residue methods return MDG_ERROR_NOT_SUPPORTED.  All
+ * workspace lives in one static device array of
+ * mdg_noalloc_workspace_bytes() bytes (256 MiB) that every GPU the library
+ * runs on holds until the process ends; a problem that needs more returns
+ * MDG_ERROR_OUT_OF_MEMORY with the size in mdg_last_error() (C untouched).
+ * Every call on one GPU shares that GPU's workspace: calls that overlap in
+ * time on one GPU (different streams or host threads) race and must be
+ * serialised by the caller.  Like mdg_gemm, it synchronises `stream` once
+ * to read the exponent spread.  (Quad-double fits up to about n = 1024.)
+ * -------------------------------------------------------------------- */
+int mdg_gemm_noalloc(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
+                     double *C, long long ldc, int method, int target_bits, mdg_gemm_info *info, void *stream);
+
+/* Size in bytes of mdg_gemm_noalloc's static workspace. */
+size_t mdg_noalloc_workspace_bytes(void);
 
 /* ----------------------------------------------------------------------
  * Batched GEMM: `count` independent problems (L in {2,4,8,16} may differ

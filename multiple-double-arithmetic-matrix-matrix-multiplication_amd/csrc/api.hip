@@ -1,5 +1,6 @@
 // api.hip -- the extern "C" entry points of libmdgemm.so (include/mdgemm.h):
 // argument validation, status codes, scratch management and dispatch.
+#include <algorithm>
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -30,11 +31,14 @@ int fill_random(int kind, int L, long long count, unsigned long long seed, int e
                 hipStream_t s);
 int batched_gemm(int count, const mdg_problem *problems, int method, int target_bits, int spread_bits,
                  int persistent, hipStream_t s);
+int noalloc_workspace(void **ptr);
 
 static thread_local char g_err[512] = "";
 static std::atomic<int> g_profiling{0};
 
-bool profiling_on() { return g_profiling.load(std::memory_order_relaxed) != 0; }
+bool arena_active();
+
+bool profiling_on() { return g_profiling.load(std::memory_order_relaxed) != 0 && !arena_active(); }
 
 static std::atomic<int> g_int8_backend{0};
 
@@ -59,8 +63,45 @@ struct ScratchBuf {
 static std::mutex g_scratch_mu;
 static std::map<std::pair<int, void *>, std::vector<ScratchBuf>> g_scratch;
 
+// The arena installed on the calling thread (mdg_gemm_noalloc), if any
+static thread_local Arena *g_arena = nullptr;
+
+void arena_begin(void *base, size_t cap, Arena *a)
+{
+    *a = Arena{};
+    a->base = (char *)base;
+    a->cap = cap;
+    g_arena = a;
+}
+
+void arena_end() { g_arena = nullptr; }
+
+bool arena_active() { return g_arena != nullptr; }
+
+bool arena_overflow(size_t *need)
+{
+    if (!g_arena || !g_arena->overflow) return false;
+    *need = g_arena->need;
+    return true;
+}
+
 void *scratch(void *stream, size_t bytes, int slot)
 {
+    if (g_arena) {
+        Arena &a = *g_arena;
+        // pieces start at 256-byte aligned device addresses
+        const uintptr_t at = (uintptr_t)a.base + a.used;
+        const size_t off = (size_t)(((at + 255) & ~(uintptr_t)255) - (uintptr_t)a.base);
+        if (a.overflow || off + bytes > a.cap) {
+            // keep counting what the whole call would need
+            a.need = std::max(a.need, off) + bytes;
+            a.used = a.need;
+            a.overflow = true;
+            return nullptr;
+        }
+        a.used = off + bytes;
+        return a.base + off;
+    }
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -117,7 +158,7 @@ static int ensure_device()
     return MDG_SUCCESS;
 }
 
-static int check_gemm_args(int L, int m, int n, int k, const void *A, long long lda, const void *B, long long ldb,
+int check_gemm_args(int L, int m, int n, int k, const void *A, long long lda, const void *B, long long ldb,
                            const void *C, long long ldc)
 {
     MDG_REQUIRE(valid_limbs(L), MDG_ERROR_INVALID_VALUE, "unsupported limb count L=%d (expected 2, 4, 8 or 16)", L);
@@ -163,7 +204,8 @@ int mdg_init(void)
     int rc = ensure_device();
     if (rc) return rc;
     MDG_HIP_CHECK(hipFree(nullptr));
-    return MDG_SUCCESS;
+    void *ws = nullptr;   // mdg_gemm_noalloc's workspace address on this device
+    return noalloc_workspace(&ws);
 }
 
 int mdg_release_workspace(void) { return release_scratch(); }

@@ -56,6 +56,20 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // buffers, so concurrent streams never alias scratch.
 void *scratch(void *stream, size_t bytes, int slot);
 
+// Workspace arena of mdg_gemm_noalloc (api.hip): between arena_begin and
+// arena_end, scratch() on this thread returns pieces of [base, base + cap)
+// and allocates nothing; arena_overflow reports the bytes a call needed
+// when they did not fit.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, need = 0;
+    bool overflow = false;
+};
+void arena_begin(void *base, size_t cap, Arena *a);
+void arena_end();
+bool arena_active();
+bool arena_overflow(size_t *need);
+
 // mdg_set_profiling / mdg_set_int8_backend state
 bool profiling_on();
 int int8_backend();

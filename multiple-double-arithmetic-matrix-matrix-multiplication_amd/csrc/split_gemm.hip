@@ -1002,7 +1002,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        const int forced = int8_backend();
+        // inside a no-allocation call only the hand-written kernels run
+        const int forced = arena_active() ? 0 : int8_backend();
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
         constexpr int BAND_G = 4;
         BandTable tab_full, tab_part;

@@ -65,6 +65,8 @@ _SIGS = {
     "mdg_vector_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _vp], _i),
     "mdg_gemm": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _i, ctypes.POINTER(GemmInfo), _vp], _i),
     "mdg_gemm_batched": ([_i, ctypes.POINTER(Problem), _i, _i, _i, _i, _vp], _i),
+    "mdg_gemm_noalloc": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _i, ctypes.POINTER(GemmInfo), _vp], _i),
+    "mdg_noalloc_workspace_bytes": ([], ctypes.c_size_t),
     "mdg_split4pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_split8pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_mixsplit2": ([_vp, _ll, _vp, _vp], _i),
@@ -161,6 +163,22 @@ def gemm(L, m, n, k, A, B, method="int8", target_bits=0, C=None, lda=None, ldb=N
     _check(lib().mdg_gemm(L, m, n, k, _ptr(A), k if lda is None else lda, _ptr(B),
                           n if ldb is None else ldb, _ptr(C), ldc, meth, target_bits,
                           ctypes.byref(info), _stream(stream)))
+    return C, info.as_dict()
+
+
+def gemm_noalloc(L, m, n, k, A, B, method="int8", target_bits=0, C=None, lda=None, ldb=None, ldc=None,
+                 stream=None):
+    """mdg_gemm_noalloc: mdg_gemm with its workspace in the library's static
+    device array (C must be given or is allocated here by torch, outside the
+    library)."""
+    ldc = n if ldc is None else ldc
+    if C is None:
+        C = empty(m * ldc * L)
+    info = GemmInfo()
+    meth = METHODS[method] if isinstance(method, str) else method
+    _check(lib().mdg_gemm_noalloc(L, m, n, k, _ptr(A), k if lda is None else lda, _ptr(B),
+                                  n if ldb is None else ldb, _ptr(C), ldc, meth, target_bits,
+                                  ctypes.byref(info), _stream(stream)))
     return C, info.as_dict()
 
 
