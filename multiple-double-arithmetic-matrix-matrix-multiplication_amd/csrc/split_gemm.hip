@@ -818,10 +818,10 @@ static int launch_band(int gb, const BandTable &tab, const int8_t *Aall, const i
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
     switch (gb) {
-    case 4: hipLaunchKernelGGL((band_i8_kernel<4, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 3: hipLaunchKernelGGL((band_i8_kernel<3, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 2: hipLaunchKernelGGL((band_i8_kernel<2, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 1: hipLaunchKernelGGL((band_i8_kernel<1, 2>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 4: hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 3: hipLaunchKernelGGL((band_i8_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 2: hipLaunchKernelGGL((band_i8_kernel<2, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 1: hipLaunchKernelGGL((band_i8_kernel<1, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
     default: set_error("band width %d", gb); return MDG_ERROR_INVALID_VALUE;
     }
     MDG_LAUNCH_CHECK("band_i8_kernel");
