@@ -98,7 +98,8 @@ int mdg_version(void);
    initialises on first use). */
 int mdg_init(void);
 
-/* Release cached scratch buffers of the current device. */
+/* Release cached scratch buffers and cached batch plans (their graphs and
+   workspaces) of the current device; waits for the streams that use them. */
 int mdg_release_workspace(void);
 
 /* Per-stage HIP-event timing of mdg_gemm (on = 1) into mdg_gemm_info.stage_ms;
@@ -181,6 +182,24 @@ size_t mdg_noalloc_workspace_bytes(void);
 int mdg_gemm_batched(int count, const mdg_problem *problems, int method,
                      int target_bits, int spread_bits, int persistent,
                      void *stream);
+
+/* Slice and residue batches size their slices from spread_bits without
+   reading the data back; the scans then check every problem on the device:
+   a problem with an Inf/NaN input limb (flag bit 0) or an exponent spread
+   of A's rows or B's columns above spread_bits (bit 1, too few slices for
+   the precision target) gets C filled with NaN instead of a silently wrong
+   result.  mdg_batched_last_flags copies the flags of the last batched call
+   on `stream` (count must match it) into flags[count], synchronising the
+   stream, and returns MDG_ERROR_NON_FINITE or MDG_ERROR_RANGE for the first
+   flagged problem (index in mdg_last_error()), else MDG_SUCCESS. */
+int mdg_batched_last_flags(void *stream, int count, int *flags);
+
+/* Bounds of the persistent-batch plan cache (process-wide): at most
+   max_plans plans and max_bytes of device memory in them; the least
+   recently used plans are freed first (defaults 32 plans, 16 GiB).  A plan
+   is keyed by the batch's shapes, pointers, method and stream, so a caller
+   that passes new buffers on every call builds a new plan each time. */
+int mdg_set_batch_plan_cache(int max_plans, long long max_bytes);
 
 /* ----------------------------------------------------------------------
  * The reference's split ("tensor core") path, piece by piece.

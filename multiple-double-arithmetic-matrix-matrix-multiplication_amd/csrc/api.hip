@@ -32,6 +32,9 @@ int fill_random(int kind, int L, long long count, unsigned long long seed, int e
 int batched_gemm(int count, const mdg_problem *problems, int method, int target_bits, int spread_bits,
                  int persistent, hipStream_t s);
 int noalloc_workspace(void **ptr);
+int release_batch_plans();
+int set_batch_plan_cache(int max_plans, long long max_bytes);
+int batched_last_flags(void *stream, int count, int *out);
 
 static thread_local char g_err[512] = "";
 static std::atomic<int> g_profiling{0};
@@ -76,7 +79,20 @@ void arena_begin(void *base, size_t cap, Arena *a)
 
 void arena_end() { g_arena = nullptr; }
 
+void arena_reset()
+{
+    if (g_arena) g_arena->used = 0;
+}
+
 bool arena_active() { return g_arena != nullptr; }
+
+bool noalloc_active() { return g_arena != nullptr && g_arena->noalloc; }
+
+// while set, scratch() (allocating path) adds every request's size, rounded
+// to 256 bytes plus 256 of alignment slack, to *g_rec
+static thread_local size_t *g_rec = nullptr;
+
+void scratch_record(size_t *counter) { g_rec = counter; }
 
 bool arena_overflow(size_t *need)
 {
@@ -102,6 +118,7 @@ void *scratch(void *stream, size_t bytes, int slot)
         a.used = off + bytes;
         return a.base + off;
     }
+    if (g_rec) *g_rec += (bytes + 255) / 256 * 256 + 256;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -208,7 +225,23 @@ int mdg_init(void)
     return noalloc_workspace(&ws);
 }
 
-int mdg_release_workspace(void) { return release_scratch(); }
+int mdg_release_workspace(void)
+{
+    int rc = release_scratch();
+    return rc ? rc : release_batch_plans();
+}
+
+int mdg_set_batch_plan_cache(int max_plans, long long max_bytes)
+{
+    return set_batch_plan_cache(max_plans, max_bytes);
+}
+
+int mdg_batched_last_flags(void *stream, int count, int *flags)
+{
+    MDG_REQUIRE(count >= 0 && (count == 0 || flags), MDG_ERROR_INVALID_VALUE, "bad count or NULL flags");
+    int rc = ensure_device();
+    return rc ? rc : batched_last_flags(stream, count, flags);
+}
 
 int mdg_set_int8_backend(int backend)
 {

@@ -59,6 +59,8 @@ struct BProb {
     int8_t *Aall, *Ball;
     int *D, *E, *F;
     double *colsum;
+    int *flag;     // the problem's flags: bit 0 non-finite input, bit 1 spread above `allow`
+    int allow;     // exponent-spread allowance of the slice plan (bits)
 };
 
 // one block's work: problem and up to three block coordinates
@@ -84,6 +86,15 @@ __global__ void b_init_kernel(const BProb *__restrict__ P, const BEnt *__restric
         p.F[c] = INT_MIN;
         p.colsum[c] = 0.0;
     }
+    if (e.x == 0 && threadIdx.x == 0) *p.flag = 0;
+}
+
+// ceil(E - log2(mean |x0|)), clamped to [0, 256]: the measured exponent
+// spread of one row or column (as the single-problem scans measure it)
+__device__ __forceinline__ int bspread(int e, double sum0, int k)
+{
+    const double sp = (double)e - log2(sum0 / (double)k);
+    return (int)fmin(256.0, fmax(0.0, ceil(sp)));
 }
 
 template <int L>
@@ -94,17 +105,31 @@ __global__ __launch_bounds__(256) void b_scan_rows_kernel(const BProb *__restric
     const int lane = threadIdx.x & 63;
     const long long r = (long long)e.x * 4 + (threadIdx.x >> 6);
     if (r >= p.m) return;
-    int emax = INT_MIN;
+    int emax = INT_MIN, bad = 0;
+    double sum0 = 0.0;
     for (int kk = lane; kk < p.k; kk += 64) {
         const double *x = p.A + (r * p.lda + kk) * L;
         double s = 0.0;
 #pragma unroll
-        for (int l = 0; l < L; ++l) s += fabs(x[l]);
+        for (int l = 0; l < L; ++l) {
+            bad |= !isfinite(x[l]);
+            s += fabs(x[l]);
+        }
         emax = max(emax, bscale_exponent(s));
+        sum0 += fabs(x[0]);
     }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) emax = max(emax, __shfl_xor(emax, o));
-    if (lane == 0) p.E[r] = (emax == INT_MIN) ? BZERO_SCALE : emax;
+    for (int o = 32; o >= 1; o >>= 1) {
+        emax = max(emax, __shfl_xor(emax, o));
+        sum0 += __shfl_xor(sum0, o);
+        bad |= __shfl_xor(bad, o);
+    }
+    if (lane == 0) {
+        p.E[r] = (emax == INT_MIN) ? BZERO_SCALE : emax;
+        int f = bad ? 1 : 0;
+        if (emax != INT_MIN && sum0 > 0.0 && bspread(emax, sum0, p.k) > p.allow) f |= 2;
+        if (f) atomicOr(p.flag, f);
+    }
 }
 
 template <int L>
@@ -113,24 +138,34 @@ __global__ __launch_bounds__(256) void b_scan_cols_kernel(const BProb *__restric
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
     __shared__ int se[4][64];
+    __shared__ double ss[4][64];
     const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
     const long long c = (long long)e.x * 64 + cl;
     const int k0 = e.y * 256, k1 = min(p.k, k0 + 256);
-    int emax = INT_MIN;
+    int emax = INT_MIN, bad = 0;
+    double sum0 = 0.0;
     if (c < p.n) {
         for (int kk = k0 + grp; kk < k1; kk += 4) {
             const double *x = p.B + ((long long)kk * p.ldb + c) * L;
             double s = 0.0;
 #pragma unroll
-            for (int l = 0; l < L; ++l) s += fabs(x[l]);
+            for (int l = 0; l < L; ++l) {
+                bad |= !isfinite(x[l]);
+                s += fabs(x[l]);
+            }
             emax = max(emax, bscale_exponent(s));
+            sum0 += fabs(x[0]);
         }
     }
     se[grp][cl] = emax;
+    ss[grp][cl] = sum0;
+    if (bad) atomicOr(p.flag, 1);
     __syncthreads();
     if (grp == 0 && c < p.n) {
         const int m4 = max(max(se[0][cl], se[1][cl]), max(se[2][cl], se[3][cl]));
+        const double s4 = (ss[0][cl] + ss[1][cl]) + (ss[2][cl] + ss[3][cl]);
         if (m4 != INT_MIN) atomicMax(&p.F[c], m4);
+        if (s4 != 0.0) atomicAdd(&p.colsum[c], s4);
     }
 }
 
@@ -139,7 +174,28 @@ __global__ void b_finish_cols_kernel(const BProb *__restrict__ P, const BEnt *__
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
     const int c = e.x * 256 + threadIdx.x;
-    if (c < p.n && p.F[c] == INT_MIN) p.F[c] = BZERO_SCALE;
+    if (c >= p.n) return;
+    if (p.F[c] == INT_MIN) {
+        p.F[c] = BZERO_SCALE;
+        return;
+    }
+    if (p.colsum[c] > 0.0 && bspread(p.F[c], p.colsum[c], p.k) > p.allow) atomicOr(p.flag, 2);
+}
+
+// Last stage: a problem flagged by the scans (non-finite input, or an
+// exponent spread beyond the allowance its slices were sized for) gets C
+// filled with NaN instead of a silently inaccurate result.  One block per
+// problem.
+__global__ void b_guard_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+{
+    const BEnt e = T[blockIdx.x];
+    const BProb &p = P[e.prob];
+    if (*p.flag == 0) return;
+    const long long nel = (long long)p.m * p.n * p.L;
+    for (long long i = threadIdx.x; i < nel; i += blockDim.x) {
+        const long long r = i / ((long long)p.n * p.L), rem = i - r * p.n * p.L;
+        p.C[r * p.ldc * p.L + rem] = __builtin_nan("");
+    }
 }
 
 // --------------------------------------------------------------- slices --
@@ -316,7 +372,8 @@ __global__ __launch_bounds__(256) void b_vector_kernel(const BProb *__restrict__
 static long long rup(long long x, long long q) { return (x + q - 1) / q * q; }
 
 struct Stage {
-    int kind;      // 0 init, 1 scan rows, 2 scan cols, 3 finish, 4 slice A, 5 slice B, 6 gemm, 7 recon, 8 vector
+    int kind;      // 0 init, 1 scan rows, 2 scan cols, 3 finish, 4 slice A, 5 slice B, 6 gemm, 7 recon,
+                   // 8 vector, 9 guard (NaN-fill flagged problems)
     int L;         // limb count of the stage's problems (0: any)
     long long first, count;   // range in the entry table
     int nw = 0;    // reconstruction: diagonal-sum words its problems need
@@ -328,9 +385,17 @@ struct Plan {
     std::vector<BProb> probs;
     std::vector<BEnt> ents;
     std::vector<Stage> stages;
+    Stage guard{9, 0, 0, 0};    // last stage of slice / residue pipelines
     BProb *dprobs = nullptr;
     BEnt *dents = nullptr;
     void *arena = nullptr;
+    int *flags = nullptr;       // per-problem flags (in the arena, or own buffer)
+    void *work = nullptr;       // loop-in-graph plans: their own per-problem workspace
+    size_t work_bytes = 0;
+    size_t bytes = 0;           // device bytes the plan holds
+    int count = 0, dev = 0;
+    hipStream_t stream = nullptr;
+    unsigned long long last_use = 0;
     hipGraphExec_t exec = nullptr;
     // GEMM stage schedules the autotuner times on the first call: deepest
     // planes first with 128- and 64-row tiles, problem-major with 128
@@ -344,6 +409,8 @@ struct Plan {
         (void)hipFree(dprobs);
         (void)hipFree(dents);
         (void)hipFree(arena);
+        (void)hipFree(work);
+        if (flags && !arena) (void)hipFree(flags);
     }
 };
 
@@ -363,6 +430,8 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
     };
     std::vector<Offs> offs(count);
     pl.probs.resize(count);
+    pl.count = count;
+    const size_t off_flags = (i8 || res) ? take((size_t)count * 4) : 0;
     for (int i = 0; i < count; ++i) {
         const mdg_problem &q = pr[i];
         BProb &b = pl.probs[i];
@@ -405,7 +474,17 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
     }
     if (bytes) {
         MDG_HIP_CHECK(hipMalloc(&pl.arena, bytes));
+        pl.bytes += bytes;
         char *base = (char *)pl.arena;
+        if (i8 || res) {
+            pl.flags = (int *)(base + off_flags);
+            // problems without entries (empty) keep flag 0 on every replay
+            MDG_HIP_CHECK(hipMemset(pl.flags, 0, (size_t)count * 4));
+            for (int i = 0; i < count; ++i) {
+                pl.probs[i].flag = pl.flags + i;
+                pl.probs[i].allow = spread;
+            }
+        }
         for (int i = 0; i < count; ++i) {
             BProb &b = pl.probs[i];
             if (!b.S) continue;
@@ -524,11 +603,21 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                     for (int y = 0; y < (b.n + 15) / 16; ++y) push(i, x, y, 0);
             });
     }
+    if (i8 || res) {
+        pl.guard.first = (long long)pl.ents.size();
+        for (int i = 0; i < count; ++i) {
+            const BProb &b = pl.probs[i];
+            if (b.m == 0 || b.n == 0 || b.k == 0) continue;
+            push(i, 0, 0, 0);
+        }
+        pl.guard.count = (long long)pl.ents.size() - pl.guard.first;
+    }
     for (const Stage &st : pl.stages)
         MDG_REQUIRE(st.count <= 2147483647ll, MDG_ERROR_NOT_SUPPORTED, "batch too large for one launch");
     MDG_HIP_CHECK(hipMalloc(&pl.dprobs, sizeof(BProb) * std::max(1, count)));
     MDG_HIP_CHECK(hipMemcpy(pl.dprobs, pl.probs.data(), sizeof(BProb) * count, hipMemcpyHostToDevice));
     MDG_HIP_CHECK(hipMalloc(&pl.dents, sizeof(BEnt) * std::max<size_t>(1, pl.ents.size())));
+    pl.bytes += sizeof(BProb) * std::max(1, count) + sizeof(BEnt) * std::max<size_t>(1, pl.ents.size());
     MDG_HIP_CHECK(
         hipMemcpy(pl.dents, pl.ents.data(), sizeof(BEnt) * pl.ents.size(), hipMemcpyHostToDevice));
     return MDG_SUCCESS;
@@ -551,6 +640,7 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 1: MDG_BY_L(b_scan_rows_kernel); break;
     case 2: MDG_BY_L(b_scan_cols_kernel); break;
     case 3: hipLaunchKernelGGL(b_finish_cols_kernel, grid, blk, 0, s, P, T); break;
+    case 9: hipLaunchKernelGGL(b_guard_kernel, grid, blk, 0, s, P, T); break;
     case 4: MDG_BY_L(b_slice_a_kernel); break;
     case 5: MDG_BY_L(b_slice_b_kernel); break;
     case 6:
@@ -591,7 +681,11 @@ static int run_plan(const Plan &pl, hipStream_t s)
         int rc = launch_stage(pl, alt ? pl.gemm_cands[pl.gemm_pick] : st, s);
         if (rc) return rc;
     }
-    return pl.rb ? residue_batch_run(pl.rb, s) : MDG_SUCCESS;
+    if (pl.rb) {
+        int rc = residue_batch_run(pl.rb, s);
+        if (rc) return rc;
+    }
+    return pl.guard.count ? launch_stage(pl, pl.guard, s) : MDG_SUCCESS;
 }
 
 // Autotune the batch schedule: run the whole pipeline once with every GEMM
@@ -639,27 +733,186 @@ static int tune_plan(Plan &pl, hipStream_t s)
     return MDG_SUCCESS;
 }
 
-static int run_loop(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s)
+__global__ void b_zero_flags_kernel(int *flags, int count)
 {
-    for (int i = 0; i < count; ++i) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) flags[i] = 0;
+}
+
+// The per-problem calls.  flags: each slice / residue problem's guard
+// flags (split_gemm.hip).  record: the largest scratch need of one problem
+// is written there (an uncaptured sizing pass).  work: a workspace of
+// work_bytes that every problem's scratch requests are carved from, from its
+// start for each problem (the calls run in stream order): a plan recorded
+// into a graph then refers to memory it owns, never to per-stream scratch
+// that a later call could grow or release.
+static int run_loop(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s,
+                    int *flags, size_t *record, void *work, size_t work_bytes)
+{
+    if (flags) {
+        hipLaunchKernelGGL(b_zero_flags_kernel, dim3((count + 255) / 256), dim3(256), 0, s, flags, count);
+        MDG_LAUNCH_CHECK("b_zero_flags_kernel");
+    }
+    Arena arena;
+    if (work) arena_begin(work, work_bytes, &arena);
+    int rc = MDG_SUCCESS;
+    size_t need = 0;
+    for (int i = 0; i < count && rc == MDG_SUCCESS; ++i) {
         const mdg_problem &p = pr[i];
         if (p.m == 0 || p.n == 0 || p.k == 0) continue;
-        int rc = (method == MDG_METHOD_VECTOR)
-                     ? vector_gemm(p.L, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, MDG_TWOPROD_FMA, s)
-                     : split_gemm(p.L, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, method, target_bits,
-                                  spread, nullptr, s);
-        if (rc) return rc;
+        size_t cur = 0;
+        if (record) scratch_record(&cur);
+        if (work) arena_reset();
+        set_guard_flag(flags ? flags + i : nullptr);
+        rc = (method == MDG_METHOD_VECTOR)
+                 ? vector_gemm(p.L, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, MDG_TWOPROD_FMA, s)
+                 : split_gemm(p.L, p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc, method, target_bits, spread,
+                              nullptr, s);
+        set_guard_flag(nullptr);
+        if (record) scratch_record(nullptr);
+        need = std::max(need, cur);
+    }
+    size_t over = 0;
+    if (work && arena_overflow(&over) && rc == MDG_SUCCESS) {
+        set_error("internal: a batch problem needs %zu bytes of workspace, more than its plan's %zu", over,
+                  work_bytes);
+        rc = MDG_ERROR_OUT_OF_MEMORY;
+    }
+    if (work) arena_end();
+    if (record) *record = need;
+    return rc;
+}
+
+// ------------------------------------------------------------ plan cache --
+// Plans are cached by batch signature (device, stream, method, precision,
+// every problem's shape and pointers): a replay is tied to the buffers it
+// was recorded with.  The cache is bounded (least recently used plans are
+// freed first) and mdg_release_workspace empties it for the device.
+static std::mutex g_plan_mu;
+static std::map<std::string, Plan *> g_plans;
+static size_t g_plan_max_bytes = size_t(16) << 30;
+static int g_plan_max_count = 32;
+static unsigned long long g_plan_clock = 0;
+
+// per (device, stream): the flags of the last batched call there
+struct LastFlags {
+    int *flags = nullptr;
+    int count = 0;
+};
+static std::map<std::pair<int, void *>, LastFlags> g_last;
+// per (device, stream): the flag buffer of non-persistent calls
+struct LoopFlags {
+    int *buf = nullptr;
+    int cap = 0;
+};
+static std::map<std::pair<int, void *>, LoopFlags> g_loop_flags;
+
+static void forget_flags_locked(const int *flags)
+{
+    for (auto it = g_last.begin(); it != g_last.end();)
+        it = (it->second.flags == flags) ? g_last.erase(it) : std::next(it);
+}
+
+static void free_plan_locked(std::map<std::string, Plan *>::iterator it)
+{
+    Plan *pl = it->second;
+    (void)hipStreamSynchronize(pl->stream);   // a replay may still run
+    forget_flags_locked(pl->flags);
+    delete pl;
+    g_plans.erase(it);
+}
+
+// evict least recently used plans of any device until the cache fits
+static void trim_plans_locked(const Plan *keep)
+{
+    for (;;) {
+        size_t bytes = 0;
+        for (auto &kv : g_plans) bytes += kv.second->bytes;
+        if ((int)g_plans.size() <= g_plan_max_count && bytes <= g_plan_max_bytes) return;
+        auto victim = g_plans.end();
+        for (auto it = g_plans.begin(); it != g_plans.end(); ++it)
+            if (it->second != keep && (victim == g_plans.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        if (victim == g_plans.end()) return;   // only the plan just built is left
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (victim->second->dev != cur) (void)hipSetDevice(victim->second->dev);
+        free_plan_locked(victim);
+        (void)hipSetDevice(cur);
+    }
+}
+
+int release_batch_plans()
+{
+    int dev = 0;
+    MDG_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    for (auto it = g_plans.begin(); it != g_plans.end();) {
+        auto next = std::next(it);
+        if (it->second->dev == dev) free_plan_locked(it);
+        it = next;
+    }
+    for (auto it = g_loop_flags.begin(); it != g_loop_flags.end();) {
+        if (it->first.first == dev) {
+            (void)hipStreamSynchronize((hipStream_t)it->first.second);
+            forget_flags_locked(it->second.buf);
+            (void)hipFree(it->second.buf);
+            it = g_loop_flags.erase(it);
+        } else {
+            ++it;
+        }
     }
     return MDG_SUCCESS;
 }
 
-static std::mutex g_plan_mu;
-static std::map<std::string, Plan *> g_plans;
+int set_batch_plan_cache(int max_plans, long long max_bytes)
+{
+    MDG_REQUIRE(max_plans >= 1 && max_bytes >= 0, MDG_ERROR_INVALID_VALUE, "bad plan cache bounds (%d, %lld)",
+                max_plans, max_bytes);
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    g_plan_max_count = max_plans;
+    g_plan_max_bytes = (size_t)max_bytes;
+    trim_plans_locked(nullptr);
+    return MDG_SUCCESS;
+}
 
-static std::string signature(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s)
+int batched_last_flags(void *stream, int count, int *out)
+{
+    int dev = 0;
+    MDG_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    auto it = g_last.find({dev, stream});
+    MDG_REQUIRE(it != g_last.end(), MDG_ERROR_INVALID_VALUE, "no batched call on this stream (or its plan was freed)");
+    MDG_REQUIRE(count == it->second.count, MDG_ERROR_INVALID_VALUE,
+                "the last batched call on this stream had %d problems, not %d", it->second.count, count);
+    if (count == 0) return MDG_SUCCESS;
+    if (!it->second.flags) {   // a vector-method batch: nothing to flag
+        for (int i = 0; i < count; ++i) out[i] = 0;
+        return MDG_SUCCESS;
+    }
+    MDG_HIP_CHECK(hipMemcpyAsync(out, it->second.flags, sizeof(int) * (size_t)count, hipMemcpyDeviceToHost,
+                                 (hipStream_t)stream));
+    MDG_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    for (int i = 0; i < count; ++i) {
+        if (out[i] & 1) {
+            set_error("problem %d: an input limb is Inf or NaN (its C was filled with NaN)", i);
+            return MDG_ERROR_NON_FINITE;
+        }
+        if (out[i] & 2) {
+            set_error("problem %d: exponent spread beyond the batch's spread allowance (its C was filled with NaN)",
+                      i);
+            return MDG_ERROR_RANGE;
+        }
+    }
+    return MDG_SUCCESS;
+}
+
+static std::string signature(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s,
+                             int dev)
 {
     std::string key;
     key.reserve(64 + sizeof(mdg_problem) * (size_t)count);
+    key.append((const char *)&dev, sizeof(dev));
     key.append((const char *)&method, sizeof(method));
     key.append((const char *)&target_bits, sizeof(target_bits));
     key.append((const char *)&spread, sizeof(spread));
@@ -672,21 +925,48 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
                  hipStream_t s)
 {
     const int spread = spread_bits < 0 ? 16 : spread_bits;
-    // zero-k problems: C = 0 (vector path writes zeros for k == 0 via the loop)
-    if (!persistent) return run_loop(count, pr, method, target_bits, spread, s);
+    int dev = 0;
+    MDG_HIP_CHECK(hipGetDevice(&dev));
+    if (!persistent) {
+        // flags of the per-problem guards in a small buffer of this stream
+        int *flags = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(g_plan_mu);
+            LoopFlags &lf = g_loop_flags[{dev, (void *)s}];
+            if (lf.cap < count) {
+                if (lf.buf) {
+                    MDG_HIP_CHECK(hipStreamSynchronize(s));
+                    forget_flags_locked(lf.buf);
+                    (void)hipFree(lf.buf);
+                    lf.buf = nullptr;
+                    lf.cap = 0;
+                }
+                MDG_HIP_CHECK(hipMalloc(&lf.buf, sizeof(int) * (size_t)std::max(count, 64)));
+                lf.cap = std::max(count, 64);
+            }
+            flags = lf.buf;
+            g_last[{dev, (void *)s}] = LastFlags{flags, count};
+        }
+        return run_loop(count, pr, method, target_bits, spread, s, flags, nullptr, nullptr, 0);
+    }
     for (int i = 0; i < count; ++i)
         MDG_REQUIRE(pr[i].k > 0 || pr[i].m == 0 || pr[i].n == 0, MDG_ERROR_NOT_SUPPORTED,
                     "persistent batches need k > 0 (problem %d)", i);
     MDG_REQUIRE(s != nullptr, MDG_ERROR_NOT_SUPPORTED, "persistent batches need a non-null stream");
 
-    const std::string key = signature(count, pr, method, target_bits, spread, s);
+    const std::string key = signature(count, pr, method, target_bits, spread, s, dev);
     std::lock_guard<std::mutex> lock(g_plan_mu);
     auto it = g_plans.find(key);
     if (it != g_plans.end()) {
+        it->second->last_use = ++g_plan_clock;
         MDG_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
+        g_last[{dev, (void *)s}] = LastFlags{it->second->flags, count};
         return MDG_SUCCESS;
     }
     Plan *pl = new Plan();
+    pl->dev = dev;
+    pl->stream = s;
+    pl->count = count;
     bool fused =
         (method == MDG_METHOD_INT8_OZAKI || method == MDG_METHOD_VECTOR || method == MDG_METHOD_INT8_RESIDUE);
     int rc = fused ? build_plan(*pl, count, pr, method, target_bits, spread) : MDG_SUCCESS;
@@ -695,6 +975,9 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         // calls inside the graph instead
         delete pl;
         pl = new Plan();
+        pl->dev = dev;
+        pl->stream = s;
+        pl->count = count;
         fused = false;
         rc = MDG_SUCCESS;
     }
@@ -702,9 +985,31 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         delete pl;
         return rc;
     }
-    // uncaptured passes size every library scratch buffer (capture must not
-    // allocate) and, for the fused pipeline, choose the GEMM tile order
-    rc = fused ? tune_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
+    if (fused) {
+        // an uncaptured pass sizes every library scratch buffer (capture must
+        // not allocate) and chooses the GEMM tile order
+        rc = tune_plan(*pl, s);
+    } else {
+        // the per-problem calls: an uncaptured pass (computing this call's
+        // results) records the largest scratch need of one problem; the plan
+        // gets a workspace of that size, which the captured calls use
+        if (hipMalloc(&pl->flags, sizeof(int) * (size_t)std::max(count, 1)) != hipSuccess) {
+            delete pl;
+            set_error("flag buffer allocation failed");
+            return MDG_ERROR_OUT_OF_MEMORY;
+        }
+        size_t need = 0;
+        rc = run_loop(count, pr, method, target_bits, spread, s, pl->flags, &need, nullptr, 0);
+        if (!rc) {
+            pl->work_bytes = std::max<size_t>(need, 256);
+            if (hipMalloc(&pl->work, pl->work_bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("workspace allocation of %zu bytes for the batch plan failed", pl->work_bytes);
+                rc = MDG_ERROR_OUT_OF_MEMORY;
+            }
+            pl->bytes = pl->work_bytes + sizeof(int) * (size_t)count;
+        }
+    }
     if (rc) {
         delete pl;
         return rc;
@@ -716,7 +1021,8 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         delete pl;
         MDG_HIP_CHECK(e);
     }
-    rc = fused ? run_plan(*pl, s) : run_loop(count, pr, method, target_bits, spread, s);
+    rc = fused ? run_plan(*pl, s)
+               : run_loop(count, pr, method, target_bits, spread, s, pl->flags, nullptr, pl->work, pl->work_bytes);
     e = hipStreamEndCapture(s, &graph);
     if (rc || e != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -730,7 +1036,10 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         delete pl;
         MDG_HIP_CHECK(e);
     }
+    pl->last_use = ++g_plan_clock;
     g_plans[key] = pl;
+    g_last[{dev, (void *)s}] = LastFlags{pl->flags, count};
+    trim_plans_locked(pl);
     return MDG_SUCCESS;   // the uncaptured pass already computed this call
 }
 

@@ -64,11 +64,19 @@ struct Arena {
     char *base = nullptr;
     size_t cap = 0, used = 0, need = 0;
     bool overflow = false;
+    bool noalloc = false;   // installed by mdg_gemm_noalloc: hand-written kernels only
 };
 void arena_begin(void *base, size_t cap, Arena *a);
 void arena_end();
+void arena_reset();   // start carving from the base again (the next problem of a loop)
 bool arena_active();
+bool noalloc_active();
 bool arena_overflow(size_t *need);
+// record the sizes of scratch() requests made on this thread (NULL: stop)
+void scratch_record(size_t *counter);
+
+// batched.hip: per-problem flags of a fixed-spread call (split_gemm.hip)
+void set_guard_flag(int *flag);
 
 // mdg_set_profiling / mdg_set_int8_backend state
 bool profiling_on();

@@ -66,6 +66,7 @@ int mdg_gemm_noalloc(int L, int m, int n, int k, const double *A, long long lda,
     if (rc) return rc;
     Arena arena;
     arena_begin(ws, NOALLOC_BYTES, &arena);
+    arena.noalloc = true;
     rc = mdg_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, info, stream);
     size_t need = 0;
     const bool over = arena_overflow(&need);

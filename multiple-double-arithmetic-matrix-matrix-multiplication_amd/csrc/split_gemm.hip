@@ -914,6 +914,32 @@ int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b)
     return plan_slices(p, target_bits) ? -1 : p.S;
 }
 
+// stats of the last scan on this thread (the guard of a fixed-spread call reads them)
+static thread_local int *g_last_stats = nullptr;
+// where the guard of the next fixed-spread call stores its flags (a batch's
+// per-problem flag, set by the batched loop; NULL: not stored)
+static thread_local int *g_guard_flag = nullptr;
+
+void set_guard_flag(int *flag) { g_guard_flag = flag; }
+
+// A call with a caller-fixed spread allowance (no read-back) checks on the
+// device what the scans measured: bit 0 = an input limb is Inf or NaN, bit 1
+// = the exponent spread of A's rows or B's columns exceeds the allowance, so
+// the slices were too few for the precision target.  A flagged result is
+// filled with NaN (no silent inaccuracy) and the flags go to *flag.
+__global__ void guard_kernel(const int *__restrict__ stats, int allow, int m, int n, double *__restrict__ C,
+                             long long ldc, int L, int *__restrict__ flag)
+{
+    const int f = (stats[1] ? 1 : 0) | ((stats[0] > allow || stats[2] > allow) ? 2 : 0);
+    if (flag && blockIdx.x == 0 && threadIdx.x == 0) *flag = f;
+    if (!f) return;
+    const long long nel = (long long)m * n * L;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (long long)gridDim.x * blockDim.x) {
+        const long long r = i / ((long long)n * L), rem = i - r * n * L;
+        C[r * ldc * L + rem] = __builtin_nan("");
+    }
+}
+
 template <int L>
 static int scan_scales_l(int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
                          int fixed_spread, hipStream_t s, int **Eo, int **Fo, int *spread_a, int *spread_b)
@@ -923,6 +949,7 @@ static int scan_scales_l(int m, int n, int k, const double *A, long long lda, co
     MDG_REQUIRE(E, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     int *F = E + m;
     int *stats = F + n;
+    g_last_stats = stats;
     double *colsum = (double *)scratch(s, sizeof(double) * (size_t)n, 1);
     MDG_REQUIRE(colsum, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, F, (long long)n, INT_MIN);
@@ -1003,7 +1030,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
         // inside a no-allocation call only the hand-written kernels run
-        const int forced = arena_active() ? 0 : int8_backend();
+        const int forced = noalloc_active() ? 0 : int8_backend();
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
         constexpr int BAND_G = 4;
         BandTable tab_full, tab_part;
@@ -1138,11 +1165,33 @@ int residue_gemm(int L, int m, int n, int k, const double *A, long long lda, con
                  double *C, long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info,
                  hipStream_t s);
 
+static int split_gemm_slices(int L, int m, int n, int k, const double *A, long long lda, const double *B,
+                             long long ldb, double *C, long long ldc, int method, int target_bits, int fixed_spread,
+                             mdg_gemm_info *info, hipStream_t s);
+
 int split_gemm(int L, int m, int n, int k, const double *A, long long lda, const double *B, long long ldb, double *C,
                long long ldc, int method, int target_bits, int fixed_spread, mdg_gemm_info *info, hipStream_t s)
 {
-    if (method == MDG_METHOD_INT8_RESIDUE || method == MDG_METHOD_F64_RESIDUE)
-        return residue_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, fixed_spread, info, s);
+    g_last_stats = nullptr;
+    int rc;
+    if (method == MDG_METHOD_INT8_RESIDUE || method == MDG_METHOD_F64_RESIDUE) {
+        rc = residue_gemm(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, fixed_spread, info, s);
+    } else {
+        rc = split_gemm_slices(L, m, n, k, A, lda, B, ldb, C, ldc, method, target_bits, fixed_spread, info, s);
+    }
+    if (rc || fixed_spread < 0) return rc;
+    MDG_REQUIRE(g_last_stats, MDG_ERROR_HIP, "internal: no scan statistics for the guard");
+    const long long nel = (long long)m * n * L;
+    hipLaunchKernelGGL(guard_kernel, dim3((unsigned)std::min<long long>(1024, (nel + 255) / 256)), dim3(256), 0, s,
+                       (const int *)g_last_stats, fixed_spread, m, n, C, ldc, L, g_guard_flag);
+    MDG_LAUNCH_CHECK("guard_kernel");
+    return MDG_SUCCESS;
+}
+
+static int split_gemm_slices(int L, int m, int n, int k, const double *A, long long lda, const double *B,
+                             long long ldb, double *C, long long ldc, int method, int target_bits, int fixed_spread,
+                             mdg_gemm_info *info, hipStream_t s)
+{
     Plan p{};
     p.method = method;
     p.L = L;

@@ -67,6 +67,8 @@ _SIGS = {
     "mdg_gemm_batched": ([_i, ctypes.POINTER(Problem), _i, _i, _i, _i, _vp], _i),
     "mdg_gemm_noalloc": ([_i, _i, _i, _i, _vp, _ll, _vp, _ll, _vp, _ll, _i, _i, ctypes.POINTER(GemmInfo), _vp], _i),
     "mdg_noalloc_workspace_bytes": ([], ctypes.c_size_t),
+    "mdg_batched_last_flags": ([_vp, _i, ctypes.POINTER(ctypes.c_int)], _i),
+    "mdg_set_batch_plan_cache": ([_i, _ll], _i),
     "mdg_split4pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_split8pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_mixsplit2": ([_vp, _ll, _vp, _vp], _i),
@@ -199,15 +201,48 @@ def gemm_batched(problems, method="int8", target_bits=0, spread_bits=-1, persist
             _side_stream = torch.cuda.Stream()
         cur = torch.cuda.current_stream()
         _side_stream.wait_stream(cur)
+        _set_last_stream(_side_stream)
         _check(lib().mdg_gemm_batched(len(problems), arr, meth, target_bits, spread_bits, persistent,
                                       ctypes.c_void_p(_side_stream.cuda_stream)))
         cur.wait_stream(_side_stream)
         return
+    _set_last_stream(stream)
     _check(lib().mdg_gemm_batched(len(problems), arr, meth, target_bits, spread_bits, persistent,
                                   _stream(stream)))
 
 
 _side_stream = None
+_last_batch_stream = None
+
+
+def _set_last_stream(stream):
+    global _last_batch_stream
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    _last_batch_stream = stream
+
+
+def batched_last_flags(count, stream=None):
+    """(status, flags) of the last batched call on `stream` (default: the
+    stream the last gemm_batched call ran on): flags[i] bit 0 =
+    non-finite input, bit 1 = exponent spread above the allowance (C of
+    that problem filled with NaN); status 0, 5 or 6 for the first flagged."""
+    arr = (ctypes.c_int * max(count, 1))()
+    if stream is None:
+        stream = _last_batch_stream   # the stream gemm_batched last ran on
+    rc = lib().mdg_batched_last_flags(_stream(stream), count, arr)
+    if rc not in (0, 5, 6):
+        _check(rc)
+    return rc, [arr[i] for i in range(count)]
+
+
+def set_batch_plan_cache(max_plans, max_bytes):
+    _check(lib().mdg_set_batch_plan_cache(max_plans, max_bytes))
+
+
+def release_workspace():
+    _check(lib().mdg_release_workspace())
 
 
 def split4pd(x, p, stream=None):

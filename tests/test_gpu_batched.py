@@ -179,3 +179,131 @@ def test_batched_padded_leading_dimensions(mdg, orc, torch_cuda, method):
         torch_cuda.cuda.synchronize()
         for i, (p, C) in enumerate(zip(probs, loop)):
             assert_bits_equal(to_host(p["C"]), C, f"{method} padded batch rep={rep} problem {i}")
+
+
+def _gen_batch(mdg, orc, shapes, seed, emin, emax):
+    probs, host = [], []
+    for i, (L, m, n, k) in enumerate(shapes):
+        A = orc.generate(orc.GEN_QD, L, m, k, seed + 2 * i, emin, emax)
+        B = orc.generate(orc.GEN_QD, L, k, n, seed + 2 * i + 1, emin, emax)
+        probs.append({"L": L, "m": m, "n": n, "k": k, "A": to_dev(A), "B": to_dev(B),
+                      "C": to_dev(np.zeros(m * n * L))})
+        host.append((L, m, n, k, A, B))
+    return probs, host
+
+
+@pytest.mark.parametrize("method,persistent", [("int8", 1), ("int8_residue", 1), ("bf16", 1), ("int8", 0),
+                                               ("int8_residue", 0)])
+def test_batched_spread_beyond_allowance_is_flagged(mdg, orc, torch_cuda, method, persistent):
+    # the slices are sized from spread_bits without a read-back; a problem
+    # whose measured exponent spread is larger (here leading exponents in
+    # [-30, 30] against an allowance of 2 bits) must not come back silently
+    # inaccurate: its C is NaN and the flags report it
+    shapes = [(2, 40, 36, 64), (4, 33, 20, 48), (2, 24, 24, 32)]
+    # leading exponent 0 everywhere: measured spread 1 bit, within the allowance
+    probs, host = _gen_batch(mdg, orc, shapes, 70, 0, 0)
+    wide, _ = _gen_batch(mdg, orc, [(4, 40, 36, 256)], 90, -30, 30)
+    probs[1] = wide[0]
+    for rep in range(2):   # build, then replay
+        for p in probs:
+            p["C"].fill_(0.0)
+        mdg.gemm_batched(probs, method=method, persistent=persistent, spread_bits=2)
+        torch_cuda.cuda.synchronize()
+        status, flags = mdg.batched_last_flags(len(probs))
+        assert status == 6 and flags[1] & 2, (rep, status, flags)
+        assert bool(torch_cuda.isnan(probs[1]["C"]).all()), "flagged problem not NaN-filled"
+        for i in (0, 2):
+            assert not bool(torch_cuda.isnan(probs[i]["C"]).any())
+            L, m, n, k, A, B = host[i]
+            err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(probs[i]["C"]))
+            assert err <= 4.0 * k * 2.0 ** (-53 * L), (i, err)
+
+
+def test_batched_default_allowance_covers_wide_exponent_ranges(mdg, orc, torch_cuda):
+    # the measured spread E - log2(mean |x0|) of a row of k entries is at most
+    # log2(k) + 1 (the mean is at least max / k), so the default 16-bit
+    # allowance covers every k <= 2^15: leading exponents spread over 40
+    # bits are not flagged and come back within the error bound
+    shapes = [(4, 48, 40, 512), (2, 64, 32, 300)]
+    probs, host = _gen_batch(mdg, orc, shapes, 110, -40, 0)
+    mdg.gemm_batched(probs, method="int8", persistent=1)
+    torch_cuda.cuda.synchronize()
+    status, flags = mdg.batched_last_flags(len(probs))
+    assert status == 0 and flags == [0, 0], flags
+    for p, (L, m, n, k, A, B) in zip(probs, host):
+        err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(p["C"]))
+        assert err <= 4.0 * k * 2.0 ** (-53 * L), err
+
+
+@pytest.mark.parametrize("method", ["int8", "int8_residue", "f64"])
+def test_batched_non_finite_input_is_flagged(mdg, orc, torch_cuda, method):
+    shapes = [(2, 20, 18, 40), (4, 17, 30, 25)]
+    probs, host = _gen_batch(mdg, orc, shapes, 130, -8, 8)
+    Ah = host[0][4].copy()
+    Ah[7] = np.inf
+    probs[0]["A"].copy_(to_dev(Ah))
+    mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=16)
+    torch_cuda.cuda.synchronize()
+    status, flags = mdg.batched_last_flags(2)
+    assert status == 5 and flags[0] & 1 and flags[1] == 0, (status, flags)
+    assert bool(torch_cuda.isnan(probs[0]["C"]).all())
+    L, m, n, k, A, B = host[1]
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(probs[1]["C"]))
+    assert err <= 4.0 * k * 2.0 ** (-53 * L), err
+
+
+def test_batched_plan_cache_is_bounded(mdg, orc, torch_cuda):
+    # three buffer sets cycled through persistent=1 twice with a two-plan
+    # cache: device memory after the second cycle equals that after the
+    # first (plans are evicted, not accumulated), every result equals the
+    # loop's, and releasing the workspace frees the plans
+    mdg.release_workspace()
+    mdg.set_batch_plan_cache(2, 1 << 40)
+    try:
+        sets = [_gen_batch(mdg, orc, [(2, 60, 50, 70), (4, 40, 40, 90)], 200 + 10 * j, -8, 8) for j in range(3)]
+        want = []
+        for probs, _ in sets:
+            mdg.gemm_batched(probs, method="int8", persistent=0)
+            torch_cuda.cuda.synchronize()
+            want.append([to_host(p["C"]).copy() for p in probs])
+        free = []
+        for cycle in range(2):
+            for (probs, _), w in zip(sets, want):
+                for p in probs:
+                    p["C"].fill_(0.0)
+                mdg.gemm_batched(probs, method="int8", persistent=1)
+                torch_cuda.cuda.synchronize()
+                for p, c in zip(probs, w):
+                    assert_bits_equal(to_host(p["C"]), c, f"cycle {cycle}")
+            free.append(torch_cuda.cuda.mem_get_info()[0])
+        assert free[1] == free[0], free
+        mdg.release_workspace()
+        assert torch_cuda.cuda.mem_get_info()[0] > free[1]
+    finally:
+        mdg.set_batch_plan_cache(32, 16 << 30)
+
+
+@pytest.mark.parametrize("method", ["bf16", "f64"])
+def test_batched_loop_plan_survives_scratch_growth_and_release(mdg, orc, torch_cuda, method):
+    # a persistent batch of a method without fused stages replays the
+    # per-problem calls from a graph; it must not depend on the per-stream
+    # scratch, which a larger call on the same stream regrows and
+    # mdg_release_workspace frees
+    probs, host = _gen_batch(mdg, orc, [(2, 40, 30, 50), (4, 36, 28, 44)], 300, -8, 8)
+    mdg.gemm_batched(probs, method=method, persistent=0)
+    torch_cuda.cuda.synchronize()
+    want = [to_host(p["C"]).copy() for p in probs]
+    mdg.gemm_batched(probs, method=method, persistent=1)   # build and capture
+    torch_cuda.cuda.synchronize()
+    L, n = 4, 512
+    X = mdg.fill_random(mdg.GEN_QD, L, n, n, 5, -8, 8)
+    mdg.gemm(L, n, n, n, X, X, method=method)                # regrows the stream's scratch
+    for rep in range(2):
+        if rep == 1:
+            mdg.release_workspace()                           # frees scratch and plans
+        for p in probs:
+            p["C"].fill_(0.0)
+        mdg.gemm_batched(probs, method=method, persistent=1)
+        torch_cuda.cuda.synchronize()
+        for p, w in zip(probs, want):
+            assert_bits_equal(to_host(p["C"]), w, f"{method} replay {rep}")
