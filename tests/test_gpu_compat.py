@@ -13,10 +13,10 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "tests", "cpp", "tc_compat_test")
 
 
+@pytest.mark.parametrize("n", [16, 64])   # 64: the size C++/TCtest.cu:7-19 runs
 @pytest.mark.parametrize("p", [2, 4, 8, 16])
-def test_compat_driver(orc, tmp_path, p):
+def test_compat_driver(orc, tmp_path, p, n):
     assert os.path.exists(EXE), "tests/cpp/tc_compat_test is not built; run __graft_entry__.build()"
-    n = 16
     pp = 4 * p
     r = subprocess.run([EXE, str(tmp_path), str(n), str(p)], capture_output=True, text=True,
                        timeout=120)

@@ -47,6 +47,19 @@ def test_vector_gemm_bit_identical(mdg, orc, torch_cuda, L, kind):
             assert_bits_equal(C, want, f"L={L} {kind} {m}x{n}x{k} tp={tp_dev}")
 
 
+def test_vector_gemm_hexa_double_64x64x256(mdg, orc, torch_cuda):
+    # hexa-double at a tile-filling shape: 4 x 4 tiles of 16, k spanning 16
+    # k-blocks, both two_prod flavours
+    L, m, n, k = 16, 64, 64, 256
+    A = orc.generate(orc.GEN_QD, L, m, k, 16064, -20, 20)
+    B = orc.generate(orc.GEN_QD, L, k, n, 16065, -20, 20)
+    dA, dB = to_dev(A), to_dev(B)
+    for tp_dev, tp_orc in ((mdg.TWOPROD_DEKKER, orc.TP_DEKKER), (mdg.TWOPROD_FMA, orc.TP_FMA)):
+        C = to_host(mdg.vector_gemm(L, m, n, k, dA, dB, twoprod=tp_dev))
+        want = orc.gemm(L, m, n, k, A, B, mode=tp_orc, nthreads=orc.num_threads())
+        assert_bits_equal(C, want, f"hd 64x64x256 tp={tp_dev}")
+
+
 def test_vector_gemm_leading_dimensions(mdg, orc, torch_cuda):
     L, m, n, k = 4, 9, 11, 13
     lda, ldb, ldc = 17, 15, 19
