@@ -45,6 +45,18 @@ ENGINE_OF_PATH = {"int8": 0, "int8_tensile": 3, "int8_tile256": 4}
 ENGINE_NAMES = {1: "rocblas", 2: "band_i8_kernel", 3: "hipblaslt", 4: "diag_i8_kernel"}
 
 
+def engine_name(path, backend):
+    """The GEMM engine a path ran: the INT8 slice planes on the band kernel,
+    the single-diagonal kernel or a vendor library; the INT8 residue products
+    on a library; FP64 and BF16 products on this library's MFMA kernels."""
+    if path in ENGINE_OF_PATH:
+        return ENGINE_NAMES.get(backend, str(backend))
+    if path == "int8_residue":
+        return {1: "rocblas", 3: "hipblaslt"}.get(backend, str(backend))
+    return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel", "bf16": "diag_bf16_kernel",
+            "vector": "vector_gemm_kernel"}.get(path, path)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +80,11 @@ def parse():
     ap.add_argument("--elementwise", action="store_true",
                     help="time the elementwise operations instead of the GEMM")
     ap.add_argument("--ew-entries", type=int, default=1 << 22)
+    ap.add_argument("--batched", action="store_true",
+                    help="time the batched workload instead: 1024 GEMMs, half double-double and half "
+                         "quad-double, m, n, k uniform in [16, 512], e in [-8, 8]; one persistent "
+                         "hipGraph-replayed pipeline against the loop of per-problem calls")
+    ap.add_argument("--batch-count", type=int, default=1024)
     return ap.parse_args()
 
 
@@ -121,6 +138,127 @@ def bench_elementwise(a):
                       "twoprod": "fma", "hbm_peak_GB_per_s": 8000, "ops": out}))
 
 
+def int8_slices(L, target_bits, spread_a, spread_b):
+    """Slice count of the INT8 slice method for a precision target and
+    exponent spreads (the library's plan: 8-bit slices, W = target + spreads
+    + ceil(log2((S+1) L)) + 2 fixed-point bits, S >= 4)."""
+    import math
+    S = 4
+    while 8 * S < target_bits + spread_a + spread_b + 2 + math.ceil(math.log2((S + 1) * L)):
+        S += 1
+    return S
+
+
+def bench_batched(a):
+    """Config 4: `batch_count` independent GEMMs (half dd, half qd; m, n, k
+    uniform in [16, 512]; leading exponents in [-8, 8]) through the fused
+    persistent pipeline (one launch per stage, recorded into a hipGraph and
+    replayed) and through the loop of per-problem calls.  value = effective
+    GFLOP/s (sum of 2 m n k) of a graph replay of the INT8-slice pipeline."""
+    import numpy as np
+    import torch
+
+    mdg = importlib.import_module(PKG)
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(404)
+    probs = []
+    for i in range(a.batch_count):
+        L = 2 if i < a.batch_count // 2 else 4
+        m, n, k = (int(v) for v in rng.integers(16, 513, 3))
+        probs.append({"L": L, "m": m, "n": n, "k": k,
+                      "A": mdg.fill_random(mdg.GEN_QD, L, m, k, 40000 + 2 * i, -8, 8),
+                      "B": mdg.fill_random(mdg.GEN_QD, L, k, n, 40001 + 2 * i, -8, 8),
+                      "C": mdg.empty(m * n * L)})
+    eff = sum(2.0 * p["m"] * p["n"] * p["k"] for p in probs)
+    spread = 16
+    # algorithmic INT8 work of the slice pipeline: S(S+1)/2 slice-pair products per problem
+    w_mfma = 0.0
+    for p in probs:
+        S = int8_slices(p["L"], 53 * p["L"], spread, spread)
+        w_mfma += S * (S + 1) / 2 * 2.0 * p["m"] * p["n"] * p["k"]
+    side = torch.cuda.Stream()
+    results = {}
+    for method in ("int8", "int8_residue"):
+        mdg.set_profiling(True)    # the plan records events around its GEMM stage
+        with torch.cuda.stream(side):
+            mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=spread, stream=side)
+        side.synchronize()
+        mdg.set_profiling(False)
+        for _ in range(a.warmup):
+            mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=spread, stream=side)
+        side.synchronize()
+        gemm_ms = 0.0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(side)
+        for _ in range(a.steps):
+            mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=spread, stream=side)
+        e1.record(side)
+        side.synchronize()
+        replay_ms = e0.elapsed_time(e1) / a.steps
+        # the GEMM stage of one more replay, from the events in the graph
+        mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=spread, stream=side)
+        gemm_ms = mdg.batched_last_gemm_ms(side)
+        status, flags = mdg.batched_last_flags(len(probs), side)
+        # the loop of per-problem calls (the baseline a caller would write)
+        mdg.gemm_batched(probs, method=method, persistent=0, spread_bits=spread, stream=side)
+        side.synchronize()
+        t0 = time.perf_counter()
+        mdg.gemm_batched(probs, method=method, persistent=0, spread_bits=spread, stream=side)
+        side.synchronize()
+        loop_ms = (time.perf_counter() - t0) * 1e3
+        results[method] = {"replay_ms": replay_ms, "loop_ms": loop_ms, "gemm_stage_ms": gemm_ms,
+                           "gflops_eff": eff / replay_ms / 1e6, "loop_gflops_eff": eff / loop_ms / 1e6,
+                           "flags_status": status, "flagged": int(sum(1 for f in flags if f))}
+        log(f"batched {method}: replay {replay_ms:.2f} ms ({eff / replay_ms / 1e6:.1f} GFLOP/s eff), "
+            f"GEMM stage {gemm_ms:.2f} ms, loop {loop_ms:.1f} ms")
+    # exact checker on sampled problems (the last replay was the residue
+    # pipeline; rerun the slice pipeline for the headline's check)
+    import pyoracle as orc  # noqa: E402  (the checker)
+    orc.build()
+    mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=spread, stream=side)
+    side.synchronize()
+    worst = 0.0
+    for i in range(0, len(probs), max(1, len(probs) // 32)):
+        p = probs[i]
+        L, m, n, k = p["L"], p["m"], p["n"], p["k"]
+        e, _ = orc.normwise_error(L, m, n, k, p["A"].cpu().numpy(), p["B"].cpu().numpy(), p["C"].cpu().numpy(),
+                                  nsamp=128, seed=i)
+        worst = max(worst, e / (4.0 * k * 2.0 ** (-53 * L)))
+    head = results["int8"]
+    achieved = w_mfma / (head["gemm_stage_ms"] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                "frac": achieved / INT8_PEAK_TOPS, "traffic": read_traffic("b_diag_i8_kernel", "cfg4"),
+                "kernel": "b_diag_i8_kernel", "launches_per_step": 1, "work_per_launch": w_mfma,
+                "avg_launch_ms": head["gemm_stage_ms"],
+                "note": "achieved = sum over problems of S(S+1)/2 * 2mnk INT8 ops / GEMM-stage time from HIP "
+                        "events recorded into the replayed graph"}
+    cpu = None
+    if not a.skip_cpu:
+        nthreads = orc.num_threads()
+        done_flops, t_cpu, i = 0.0, 0.0, 0
+        while t_cpu < a.cpu_seconds and i < len(probs):
+            p = probs[i]
+            Ah, Bh = p["A"].cpu().numpy(), p["B"].cpu().numpy()
+            t0 = time.perf_counter()
+            orc.gemm(p["L"], p["m"], p["n"], p["k"], Ah, Bh, mode=orc.TP_DEKKER, nthreads=nthreads)
+            t_cpu += time.perf_counter() - t0
+            done_flops += 2.0 * p["m"] * p["n"] * p["k"]
+            i += 1
+        cpu = {"value": done_flops / t_cpu / 1e9, "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+               "sample": f"the first {i} of {len(probs)} problems through the oracle's restatement of the "
+                         f"reference's host GEMM (C++/TCcuda.cu:320-372, Dekker two_prod), {t_cpu:.1f} s"}
+    line = {"metric": "effective batched multi-double GEMM GFLOP/s: 1024 GEMMs (dd/qd, m,n,k in U{16..512}) "
+                      "in one persistent hipGraph replay",
+            "value": head["gflops_eff"], "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": head["replay_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int8", "data": "synthetic",
+            "config": {"workload": f"{len(probs)} GEMMs, first half double-double, second half quad-double, "
+                                   f"m, n, k uniform in [16, 512] (seed 404), leading exponents in [-8, 8]",
+                       "effective_flops": eff, "spread_bits": spread},
+            "paths": results, "max_error_over_bound": worst, "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -144,6 +282,9 @@ def main():
         sys.exit(2)
     if a.elementwise:
         bench_elementwise(a)
+        return
+    if a.batched:
+        bench_batched(a)
         return
     import numpy as np
     import torch
@@ -199,7 +340,7 @@ def main():
                          "slices": info["slices"], "slice_bits": info["slice_bits"],
                          "pair_gemms": info["pair_gemms"], "total_bits": info["total_bits"],
                          "spread": [info["spread_a"], info["spread_b"]], "steps": steps,
-                         "engine": ENGINE_NAMES.get(info["backend"], "vector") if path != "vector" else "vector"}
+                         "engine": engine_name(path, info["backend"])}
         log(f"{path}: {ms:.2f} ms/step, {eff_flops / ms / 1e6:.1f} GFLOP/s effective, "
             f"stages {np.round(stage, 3).tolist()} ms")
     mdg.set_profiling(False)
@@ -317,7 +458,22 @@ def main():
     ratios = {"int8_over_f64": ratio("int8", "f64"), "f64_over_vector": ratio("f64", "vector"),
               "int8_residue_over_int8": ratio("int8_residue", "int8"),
               "int8_residue_over_f64_residue": ratio("int8_residue", "f64_residue"),
-              "f64_residue_over_vector": ratio("f64_residue", "vector")}
+              "f64_residue_over_vector": ratio("f64_residue", "vector"),
+              "int8_over_int8_tensile": ratio("int8", "int8_tensile")}
+    # the two speedup targets, with the pair of paths that meets each: INT8
+    # over FP64-MFMA >= 3, FP64-MFMA over the vector GEMM >= 5; slices and
+    # residues are both exact fixed-point products of the same inputs
+    targets = {}
+    for name, goal, pairs in (("int8_over_f64", 3.0, (("slices", "int8", "f64"),
+                                                      ("residues", "int8_residue", "f64_residue"))),
+                              ("f64_over_vector", 5.0, (("slices", "f64", "vector"),
+                                                        ("residues", "f64_residue", "vector")))):
+        t = {"target": goal}
+        for label, x, y in pairs:
+            r = ratio(x, y)
+            t[label] = {"paths": [x, y], "ratio": r, "met": (r >= goal) if r is not None else None}
+        targets[name] = t
+    ratios["targets"] = targets
     line = {
         "metric": "effective qd-GEMM GFLOP/s at n=4096: INT8-slice vs FP64-MFMA split vs vector, same err",
         "value": head["gflops_eff"],

@@ -194,6 +194,12 @@ int mdg_gemm_batched(int count, const mdg_problem *problems, int method,
    flagged problem (index in mdg_last_error()), else MDG_SUCCESS. */
 int mdg_batched_last_flags(void *stream, int count, int *flags);
 
+/* Duration in ms of the GEMM stage (the one launch of INT8 tiles over every
+   problem) of the last replay on `stream` of a fused persistent batch whose
+   plan was built while mdg_set_profiling(1) was on (HIP events recorded
+   into its graph); synchronises on the stage's end. */
+int mdg_batched_last_gemm_ms(void *stream, float *ms);
+
 /* Bounds of the persistent-batch plan cache (process-wide): at most
    max_plans plans and max_bytes of device memory in them; the least
    recently used plans are freed first (defaults 32 plans, 16 GiB).  A plan

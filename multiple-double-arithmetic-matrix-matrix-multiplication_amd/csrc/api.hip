@@ -35,6 +35,7 @@ int noalloc_workspace(void **ptr);
 int release_batch_plans();
 int set_batch_plan_cache(int max_plans, long long max_bytes);
 int batched_last_flags(void *stream, int count, int *out);
+int batched_last_gemm_ms(void *stream, float *ms);
 
 static thread_local char g_err[512] = "";
 static std::atomic<int> g_profiling{0};
@@ -234,6 +235,13 @@ int mdg_release_workspace(void)
 int mdg_set_batch_plan_cache(int max_plans, long long max_bytes)
 {
     return set_batch_plan_cache(max_plans, max_bytes);
+}
+
+int mdg_batched_last_gemm_ms(void *stream, float *ms)
+{
+    MDG_REQUIRE(ms, MDG_ERROR_INVALID_VALUE, "NULL ms");
+    int rc = ensure_device();
+    return rc ? rc : batched_last_gemm_ms(stream, ms);
 }
 
 int mdg_batched_last_flags(void *stream, int count, int *flags)

@@ -43,7 +43,7 @@ int plan_int8_slices(int L, int target_bits, int spread_a, int spread_b);
 struct ResBatch;
 int residue_batch_build(int count, const mdg_problem *pr, int *const *E, int *const *F, int target_bits, int spread,
                         ResBatch **out);
-int residue_batch_run(const ResBatch *rb, hipStream_t s);
+int residue_batch_run(const ResBatch *rb, hipStream_t s, hipEvent_t g0, hipEvent_t g1);
 int residue_batch_candidates(const ResBatch *rb);
 void residue_batch_select(ResBatch *rb, int c);
 void residue_batch_free(ResBatch *rb);
@@ -396,6 +396,7 @@ struct Plan {
     int count = 0, dev = 0;
     hipStream_t stream = nullptr;
     unsigned long long last_use = 0;
+    hipEvent_t gev[2] = {nullptr, nullptr};   // around the GEMM stage (built with profiling on)
     hipGraphExec_t exec = nullptr;
     // GEMM stage schedules the autotuner times on the first call: deepest
     // planes first with 128- and 64-row tiles, problem-major with 128
@@ -411,6 +412,8 @@ struct Plan {
         (void)hipFree(arena);
         (void)hipFree(work);
         if (flags && !arena) (void)hipFree(flags);
+        for (auto &e : gev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -678,11 +681,13 @@ static int run_plan(const Plan &pl, hipStream_t s)
 {
     for (const Stage &st : pl.stages) {
         const bool alt = st.kind == 6 && pl.gemm_pick >= 0;
+        if (st.kind == 6 && pl.gev[0]) MDG_HIP_CHECK(hipEventRecord(pl.gev[0], s));
         int rc = launch_stage(pl, alt ? pl.gemm_cands[pl.gemm_pick] : st, s);
         if (rc) return rc;
+        if (st.kind == 6 && pl.gev[1]) MDG_HIP_CHECK(hipEventRecord(pl.gev[1], s));
     }
     if (pl.rb) {
-        int rc = residue_batch_run(pl.rb, s);
+        int rc = residue_batch_run(pl.rb, s, pl.gev[0], pl.gev[1]);
         if (rc) return rc;
     }
     return pl.guard.count ? launch_stage(pl, pl.guard, s) : MDG_SUCCESS;
@@ -798,6 +803,7 @@ static unsigned long long g_plan_clock = 0;
 struct LastFlags {
     int *flags = nullptr;
     int count = 0;
+    const Plan *plan = nullptr;   // persistent calls: the plan replayed
 };
 static std::map<std::pair<int, void *>, LastFlags> g_last;
 // per (device, stream): the flag buffer of non-persistent calls
@@ -807,17 +813,17 @@ struct LoopFlags {
 };
 static std::map<std::pair<int, void *>, LoopFlags> g_loop_flags;
 
-static void forget_flags_locked(const int *flags)
+static void forget_flags_locked(const int *flags, const Plan *plan = nullptr)
 {
     for (auto it = g_last.begin(); it != g_last.end();)
-        it = (it->second.flags == flags) ? g_last.erase(it) : std::next(it);
+        it = (it->second.flags == flags || (plan && it->second.plan == plan)) ? g_last.erase(it) : std::next(it);
 }
 
 static void free_plan_locked(std::map<std::string, Plan *>::iterator it)
 {
     Plan *pl = it->second;
     (void)hipStreamSynchronize(pl->stream);   // a replay may still run
-    forget_flags_locked(pl->flags);
+    forget_flags_locked(pl->flags, pl);
     delete pl;
     g_plans.erase(it);
 }
@@ -907,6 +913,20 @@ int batched_last_flags(void *stream, int count, int *out)
     return MDG_SUCCESS;
 }
 
+int batched_last_gemm_ms(void *stream, float *ms)
+{
+    int dev = 0;
+    MDG_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    auto it = g_last.find({dev, stream});
+    MDG_REQUIRE(it != g_last.end() && it->second.plan && it->second.plan->gev[0], MDG_ERROR_NOT_SUPPORTED,
+                "the last batched call on this stream was no fused persistent batch built with profiling on");
+    const Plan *pl = it->second.plan;
+    MDG_HIP_CHECK(hipEventSynchronize(pl->gev[1]));
+    MDG_HIP_CHECK(hipEventElapsedTime(ms, pl->gev[0], pl->gev[1]));
+    return MDG_SUCCESS;
+}
+
 static std::string signature(int count, const mdg_problem *pr, int method, int target_bits, int spread, hipStream_t s,
                              int dev)
 {
@@ -960,7 +980,7 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
     if (it != g_plans.end()) {
         it->second->last_use = ++g_plan_clock;
         MDG_HIP_CHECK(hipGraphLaunch(it->second->exec, s));
-        g_last[{dev, (void *)s}] = LastFlags{it->second->flags, count};
+        g_last[{dev, (void *)s}] = LastFlags{it->second->flags, count, it->second};
         return MDG_SUCCESS;
     }
     Plan *pl = new Plan();
@@ -986,6 +1006,16 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
         return rc;
     }
     if (fused) {
+        if (profiling_on() && method != MDG_METHOD_VECTOR) {
+            // HIP events around the GEMM stage, recorded into the graph
+            for (auto &ev : pl->gev)
+                if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
+            if (!pl->gev[0] || !pl->gev[1]) {
+                for (auto &ev : pl->gev)
+                    if (ev) (void)hipEventDestroy(ev);
+                pl->gev[0] = pl->gev[1] = nullptr;
+            }
+        }
         // an uncaptured pass sizes every library scratch buffer (capture must
         // not allocate) and chooses the GEMM tile order
         rc = tune_plan(*pl, s);
@@ -1038,7 +1068,7 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
     }
     pl->last_use = ++g_plan_clock;
     g_plans[key] = pl;
-    g_last[{dev, (void *)s}] = LastFlags{pl->flags, count};
+    g_last[{dev, (void *)s}] = LastFlags{pl->flags, count, pl};
     trim_plans_locked(pl);
     return MDG_SUCCESS;   // the uncaptured pass already computed this call
 }

@@ -1695,11 +1695,12 @@ static int b_res_launch(const ResBatch::St &st, const ResTab &t, const RProb *P,
     return MDG_ERROR_RANGE;
 }
 
-int residue_batch_run(const ResBatch *rb, hipStream_t s)
+int residue_batch_run(const ResBatch *rb, hipStream_t s, hipEvent_t g0, hipEvent_t g1)
 {
     for (const auto &st : rb->stages) {
         const REnt *T = rb->dents + st.first;
         int rc = MDG_SUCCESS;
+        if (st.kind == 2 && g0) MDG_HIP_CHECK(hipEventRecord(g0, s));
         if (st.kind == 2) {
             const ResBatch::St &g = rb->gemm_cands[rb->gemm_pick];
             const REnt *TG = rb->dents + g.first;
@@ -1719,6 +1720,7 @@ int residue_batch_run(const ResBatch *rb, hipStream_t s)
         }
         if (rc) return rc;
         MDG_LAUNCH_CHECK("residue batch stage");
+        if (st.kind == 2 && g1) MDG_HIP_CHECK(hipEventRecord(g1, s));
     }
     return MDG_SUCCESS;
 }

@@ -783,8 +783,11 @@ enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
 
 // Band items: bands of G diagonals, each band's t-range cut into chunks of
 // Tc slices, heaviest first (work = slice-pair blocks, padding steps
-// included).  Bands of exactly G diagonals go to `full`; the last band, when
-// S is not a multiple of G, to `part` (its own kernel instance, *gpart wide).
+// included).  When S is not a multiple of G the narrower band (its own
+// kernel instance, *gpart wide, in `part`) holds the cheapest diagonals,
+// 0 .. S%G - 1 (one slice pair for diagonal 0): a narrow band reuses its
+// operands less, so it should carry little work.  Bands of exactly G
+// diagonals go to `full`.
 static bool build_band_tables(int S, int G, int Tc, BandTable &full, BandTable &part, int *gpart)
 {
     struct W {
@@ -793,8 +796,11 @@ static bool build_band_tables(int S, int G, int Tc, BandTable &full, BandTable &
     };
     std::vector<W> v[2];
     *gpart = S % G;
-    for (int g0 = 0; g0 < S; g0 += G) {
-        const int gb = std::min(G, S - g0);
+    std::vector<std::pair<int, int>> bands;   // (first diagonal, width)
+    if (*gpart) bands.push_back({0, *gpart});
+    for (int g0 = *gpart; g0 < S; g0 += G) bands.push_back({g0, G});
+    for (const auto &bd : bands) {
+        const int g0 = bd.first, gb = bd.second;
         for (int t0 = 0; t0 < g0 + gb; t0 += Tc) {
             const int t1 = std::min(t0 + Tc, g0 + gb);
             const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + gb - 1 : g0 + gb - t0;

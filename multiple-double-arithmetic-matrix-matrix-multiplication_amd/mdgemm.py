@@ -69,6 +69,7 @@ _SIGS = {
     "mdg_noalloc_workspace_bytes": ([], ctypes.c_size_t),
     "mdg_batched_last_flags": ([_vp, _i, ctypes.POINTER(ctypes.c_int)], _i),
     "mdg_set_batch_plan_cache": ([_i, _ll], _i),
+    "mdg_batched_last_gemm_ms": ([_vp, ctypes.POINTER(ctypes.c_float)], _i),
     "mdg_split4pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_split8pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_mixsplit2": ([_vp, _ll, _vp, _vp], _i),
@@ -235,6 +236,16 @@ def batched_last_flags(count, stream=None):
     if rc not in (0, 5, 6):
         _check(rc)
     return rc, [arr[i] for i in range(count)]
+
+
+def batched_last_gemm_ms(stream=None):
+    """GEMM-stage time of the last replay of a fused persistent batch built
+    with profiling on (see mdg_batched_last_gemm_ms)."""
+    ms = ctypes.c_float(0.0)
+    if stream is None:
+        stream = _last_batch_stream
+    _check(lib().mdg_batched_last_gemm_ms(_stream(stream), ctypes.byref(ms)))
+    return ms.value
 
 
 def set_batch_plan_cache(max_plans, max_bytes):
