@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "i8band.hpp"
 #include "i8tile.hpp"
 #include "md_arith.hpp"
 #include "recon.hpp"
@@ -199,83 +200,56 @@ __global__ void b_guard_kernel(const BProb *__restrict__ P, const BEnt *__restri
 }
 
 // --------------------------------------------------------------- slices --
-template <int L>
-__global__ __launch_bounds__(256) void b_slice_a_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+// slices in the blocked layout of the band kernel (i8band.hpp): thread per
+// entry (r, kk) of the padded mp x kp domain (A) or np x kp (B, "row" r =
+// column r of B), kk fastest; slice t lands in block (r / 128, kk / 128, t)
+template <int L, bool TRANS>
+__global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
     constexpr int NW = FixedWords<L>::NW;
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
+    const int rows = TRANS ? p.n : p.m, rp = TRANS ? p.np : p.mp;
     const long long idx = (long long)e.x * 256 + threadIdx.x;
-    if (idx >= (long long)p.mp * p.kp) return;
-    const long long r = idx / p.kp;
-    const int kk = (int)(idx - r * p.kp);
-    int8_t *o = p.Aall + r * ((long long)p.S * p.kp) + kk;
-    if (r >= p.m || kk >= p.k || p.E[r] == BZERO_SCALE) {
-        for (int t = 0; t < p.S; ++t) o[(long long)t * p.kp] = 0;
+    if (idx >= (long long)rp * p.kp) return;
+    const int r = (int)(idx / p.kp);
+    const int kk = (int)(idx - (long long)r * p.kp);
+    int8_t *o = (TRANS ? p.Ball : p.Aall) + bnd_block(r >> 7, kk >> 7, 0, p.S, p.kp >> 7) + bnd_pos(r & 127, kk & 127);
+    const int ex = TRANS ? p.F[r < rows ? r : 0] : p.E[r < rows ? r : 0];
+    if (r >= rows || kk >= p.k || ex == BZERO_SCALE) {
+        for (int t = 0; t < p.S; ++t) o[(long long)t * BND_BLK] = 0;
         return;
     }
     double x[L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = p.A[(r * p.lda + kk) * L + l];
+    for (int l = 0; l < L; ++l)
+        x[l] = TRANS ? p.B[((long long)kk * p.ldb + r) * L + l] : p.A[((long long)r * p.lda + kk) * L + l];
     uint32_t N[NW];
-    to_fixed<L, NW>(x, p.S * p.w - 2 - p.E[r], N);
-    fixed_digits<NW>(N, p.S, p.w, [&](int t, int d) { o[(long long)t * p.kp] = (int8_t)d; });
+    to_fixed<L, NW>(x, p.S * p.w - 2 - ex, N);
+    fixed_digits<NW>(N, p.S, p.w, [&](int t, int d) { o[(long long)t * BND_BLK] = (int8_t)d; });
 }
 
-template <int L>
-__global__ __launch_bounds__(256) void b_slice_b_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
-{
-    constexpr int NW = FixedWords<L>::NW;
-    const BEnt e = T[blockIdx.x];
-    const BProb &p = P[e.prob];
-    const long long idx = (long long)e.x * 256 + threadIdx.x;
-    if (idx >= (long long)p.np * p.kp) return;
-    const long long c = idx / p.kp;
-    const int kk = (int)(idx - c * p.kp);
-    int8_t *o = p.Ball + c * ((long long)p.S * p.kp) + kk;
-    if (c >= p.n || kk >= p.k || p.F[c] == BZERO_SCALE) {
-        for (int u = 0; u < p.S; ++u) o[(long long)u * p.kp] = 0;
-        return;
-    }
-    double x[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = p.B[((long long)kk * p.ldb + c) * L + l];
-    uint32_t N[NW];
-    to_fixed<L, NW>(x, p.S * p.w - 2 - p.F[c], N);
-    fixed_digits<NW>(N, p.S, p.w, [&](int u, int d) { o[(long long)(p.S - 1 - u) * p.kp] = (int8_t)d; });
-}
-
-// --------------------------------------------------- INT8 diagonal tiles --
-// Entry (prob, tile_m, tile_n, plane): a TM x TM tile (i8tile.hpp) of one
-// K-chunk plane of one diagonal.  Plane numbering as in the single-problem
-// path: ascending diagonal g, chunk c.
+// ------------------------------------------------------- INT8 band tiles --
+// INT32 plane count of diagonal g: its slice range cut into chunks of
+// kmax / kp slices (plane numbering of the single-problem path)
 __device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
 
-template <int TM>
-__global__ __launch_bounds__(256) void b_diag_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
+// Entry (prob, tile row, tile column, item): one band item (i8band.hpp) of
+// one 128 x 128 tile of one problem; the item packed as g0 | t0 << 8 |
+// t1 << 16, the band width is the launch's G.  An item runs nkb k-blocks
+// times its slice steps, so every workgroup is deep enough to keep its copy
+// pipeline full, however shallow the problem.
+template <int G>
+__global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[TM * I8T_LDR];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[TM * I8T_LDR];
+    __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, 2>()];
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
-    const int S = p.S, kp = p.kp, kmax = p.kmax;
-    // plane e.z -> (g, c)
-    int g = 0, first = 0;
-    while (first + b_chunks(g, kp, kmax) <= e.z) {
-        first += b_chunks(g, kp, kmax);
-        ++g;
-    }
-    const int c = e.z - first;
-    const int k0 = c * kmax, k1 = min((g + 1) * kp, k0 + kmax);
-    const long long ld = (long long)S * kp;
-    const int8_t *Ab = p.Aall + (long long)e.x * TM * ld + k0;
-    const int8_t *Bb = p.Ball + (long long)e.y * TM * ld + (long long)(S - 1 - g) * kp + k0;
-    i4t acc[I8Tile<TM>::IM][I8Tile<TM>::IM];
-    i8_tile<TM, I8Tile<TM>::RING>(Ab, ld, Bb, ld, k1 - k0, As, Bs, acc);
-    int *plane = p.D + (long long)e.z * p.m * p.n;
-    const int n = p.n;
-    i8_tile_each<TM>(acc, (long long)e.x * TM, (long long)e.y * TM, p.m, n,
-                     [&](long long r, long long col, int v) { plane[r * n + col] = v; });
+    const int g0 = e.z & 0xff, t0 = (e.z >> 8) & 0xff, t1 = (e.z >> 16) & 0xff;
+    const int nkb = p.kp >> 7, Tc = p.kmax / p.kp;
+    const long long tstride = (long long)nkb * p.S * BND_BLK;
+    band_run<G, 2, 1>(lds, p.Aall + e.x * tstride, p.Ball + e.y * tstride, p.S, nkb, Tc, g0, t0, t1,
+                      (long long)e.x * BND_TM, (long long)e.y * BND_TM, p.m, p.n, p.D);
 }
 
 // ------------------------------------------------------- reconstruction --
@@ -379,6 +353,7 @@ struct Stage {
     int nw = 0;    // reconstruction: diagonal-sum words its problems need
     bool chunked = false;   // reconstruction: some problem has K-chunk planes
     int tm = 128;  // GEMM: tile rows and columns
+    int g = 0;     // GEMM: band width of the band-kernel launch
 };
 
 struct Plan {
@@ -398,10 +373,6 @@ struct Plan {
     unsigned long long last_use = 0;
     hipEvent_t gev[2] = {nullptr, nullptr};   // around the GEMM stage (built with profiling on)
     hipGraphExec_t exec = nullptr;
-    // GEMM stage schedules the autotuner times on the first call: deepest
-    // planes first with 128- and 64-row tiles, problem-major with 128
-    std::vector<Stage> gemm_cands;
-    int gemm_pick = -1;           // -1: the stage as built
     ResBatch *rb = nullptr;       // INT8 residue stages after the scans
     ~Plan()
     {
@@ -465,7 +436,10 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         b.kp = (int)rup(q.k, 128);
         b.mp = (int)rup(q.m, 128);
         b.np = (int)rup(q.n, 128);
-        b.kmax = (b.kp <= 131071) ? (131071 / b.kp) * b.kp : 130944;
+        // the band kernel cuts a diagonal only at slice boundaries: one slice
+        // pair must fit an INT32 plane (the per-problem loop handles deeper k)
+        MDG_REQUIRE(b.kp <= 131071 && b.S < 256, MDG_ERROR_NOT_SUPPORTED, "problem %d too deep for the fused stages", i);
+        b.kmax = (131071 / b.kp) * b.kp;
         b.nplanes = 0;
         for (int g = 0; g < b.S; ++g) b.nplanes += ((g + 1) * b.kp + b.kmax - 1) / b.kmax;
         offs[i].aall = take((size_t)b.mp * b.S * b.kp);
@@ -553,38 +527,39 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 const long long nb = ((long long)b.np * b.kp + 255) / 256;
                 for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
             });
-        // GEMM tiles: deepest planes of every problem first, so long blocks
-        // start early (128-row tiles: the stage as built; 64-row tiles: a
-        // candidate), or problem-major (a candidate)
-        auto deep_first = [&](int tm) {
-            Stage st{6, 0, (long long)pl.ents.size(), 0};
-            st.tm = tm;
-            int maxp = 0;
-            for (const BProb &b : pl.probs) maxp = std::max(maxp, b.nplanes);
-            for (int z = maxp - 1; z >= 0; --z)
-                for (int i = 0; i < count; ++i) {
-                    const BProb &b = pl.probs[i];
-                    if (b.S <= 0 || z >= b.nplanes || b.m == 0 || b.n == 0 || b.k == 0) continue;
-                    for (int x = 0; x < b.mp / tm; ++x)
-                        for (int y = 0; y < b.np / tm; ++y) push(i, x, y, z);
-                }
-            st.count = (long long)pl.ents.size() - st.first;
-            return st;
-        };
-        {
-            Stage st = deep_first(128);
-            if (st.count) pl.stages.push_back(st);
-            pl.gemm_cands.push_back(deep_first(64));
-            Stage alt{6, 0, (long long)pl.ents.size(), 0};
+        // GEMM: one band-kernel launch per band width; items of all problems
+        // heaviest first (work = slice-pair blocks x tiles), their tiles
+        // next to each other
+        for (int G = 4; G >= 1; --G) {
+            struct Item {
+                int prob, code;
+                long long work;
+            };
+            std::vector<Item> items;
             for (int i = 0; i < count; ++i) {
                 const BProb &b = pl.probs[i];
                 if (b.S <= 0 || b.m == 0 || b.n == 0 || b.k == 0) continue;
-                for (int z = b.nplanes - 1; z >= 0; --z)
-                    for (int x = 0; x < b.mp / 128; ++x)
-                        for (int y = 0; y < b.np / 128; ++y) push(i, x, y, z);
+                std::vector<BandWork> full, part;
+                int gpart = 0;
+                band_items(b.S, 4, b.kmax / b.kp, full, part, &gpart);
+                const std::vector<BandWork> &v = (G == 4) ? full : part;
+                if (G != 4 && gpart != G) continue;
+                const long long tiles = (long long)(b.mp / 128) * (b.np / 128);
+                for (const BandWork &w : v)
+                    items.push_back(Item{i, w.it.g0 | (w.it.t0 << 8) | (w.it.t1 << 16),
+                                         w.work * tiles * (b.kp / 128)});
             }
-            alt.count = (long long)pl.ents.size() - alt.first;
-            pl.gemm_cands.push_back(alt);
+            if (items.empty()) continue;
+            std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.work > y.work; });
+            Stage st{6, 0, (long long)pl.ents.size(), 0};
+            st.g = G;
+            for (const Item &it : items) {
+                const BProb &b = pl.probs[it.prob];
+                for (int x = 0; x < b.mp / 128; ++x)
+                    for (int y = 0; y < b.np / 128; ++y) push(it.prob, x, y, it.code);
+            }
+            st.count = (long long)pl.ents.size() - st.first;
+            pl.stages.push_back(st);
         }
         for (int L : Ls) {
             const size_t before = pl.stages.size();
@@ -644,13 +619,29 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 2: MDG_BY_L(b_scan_cols_kernel); break;
     case 3: hipLaunchKernelGGL(b_finish_cols_kernel, grid, blk, 0, s, P, T); break;
     case 9: hipLaunchKernelGGL(b_guard_kernel, grid, blk, 0, s, P, T); break;
-    case 4: MDG_BY_L(b_slice_a_kernel); break;
-    case 5: MDG_BY_L(b_slice_b_kernel); break;
+    case 4:
+        switch (st.L) {
+        case 2: hipLaunchKernelGGL((b_slice_kernel<2, false>), grid, blk, 0, s, P, T); break;
+        case 4: hipLaunchKernelGGL((b_slice_kernel<4, false>), grid, blk, 0, s, P, T); break;
+        case 8: hipLaunchKernelGGL((b_slice_kernel<8, false>), grid, blk, 0, s, P, T); break;
+        default: hipLaunchKernelGGL((b_slice_kernel<16, false>), grid, blk, 0, s, P, T); break;
+        }
+        break;
+    case 5:
+        switch (st.L) {
+        case 2: hipLaunchKernelGGL((b_slice_kernel<2, true>), grid, blk, 0, s, P, T); break;
+        case 4: hipLaunchKernelGGL((b_slice_kernel<4, true>), grid, blk, 0, s, P, T); break;
+        case 8: hipLaunchKernelGGL((b_slice_kernel<8, true>), grid, blk, 0, s, P, T); break;
+        default: hipLaunchKernelGGL((b_slice_kernel<16, true>), grid, blk, 0, s, P, T); break;
+        }
+        break;
     case 6:
-        if (st.tm == 64)
-            hipLaunchKernelGGL(b_diag_i8_kernel<64>, grid, blk, 0, s, P, T);
-        else
-            hipLaunchKernelGGL(b_diag_i8_kernel<128>, grid, blk, 0, s, P, T);
+        switch (st.g) {
+        case 4: hipLaunchKernelGGL(b_band_i8_kernel<4>, grid, blk, 0, s, P, T); break;
+        case 3: hipLaunchKernelGGL(b_band_i8_kernel<3>, grid, blk, 0, s, P, T); break;
+        case 2: hipLaunchKernelGGL(b_band_i8_kernel<2>, grid, blk, 0, s, P, T); break;
+        default: hipLaunchKernelGGL(b_band_i8_kernel<1>, grid, blk, 0, s, P, T); break;
+        }
         break;
     case 7: {
         auto rec = [&](auto lc) -> int {
@@ -679,13 +670,15 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
 
 static int run_plan(const Plan &pl, hipStream_t s)
 {
+    bool in_gemm = false;
     for (const Stage &st : pl.stages) {
-        const bool alt = st.kind == 6 && pl.gemm_pick >= 0;
-        if (st.kind == 6 && pl.gev[0]) MDG_HIP_CHECK(hipEventRecord(pl.gev[0], s));
-        int rc = launch_stage(pl, alt ? pl.gemm_cands[pl.gemm_pick] : st, s);
+        if (st.kind == 6 && !in_gemm && pl.gev[0]) MDG_HIP_CHECK(hipEventRecord(pl.gev[0], s));
+        if (st.kind != 6 && in_gemm && pl.gev[1]) MDG_HIP_CHECK(hipEventRecord(pl.gev[1], s));
+        in_gemm = st.kind == 6;
+        int rc = launch_stage(pl, st, s);
         if (rc) return rc;
-        if (st.kind == 6 && pl.gev[1]) MDG_HIP_CHECK(hipEventRecord(pl.gev[1], s));
     }
+    if (in_gemm && pl.gev[1]) MDG_HIP_CHECK(hipEventRecord(pl.gev[1], s));
     if (pl.rb) {
         int rc = residue_batch_run(pl.rb, s, pl.gev[0], pl.gev[1]);
         if (rc) return rc;
@@ -693,16 +686,12 @@ static int run_plan(const Plan &pl, hipStream_t s)
     return pl.guard.count ? launch_stage(pl, pl.guard, s) : MDG_SUCCESS;
 }
 
-// Autotune the batch schedule: run the whole pipeline once with every GEMM
-// stage candidate (tile order and tile size for the slice planes, tile size
-// for the residue planes) and keep the fastest; all produce identical
-// results.
+// Autotune the residue batch's GEMM tiles: run the whole pipeline once with
+// every candidate (128- or 64-row tiles) and keep the fastest; all produce
+// identical results.  (The slice pipeline's band stage has one schedule.)
 static int tune_plan(Plan &pl, hipStream_t s)
 {
-    const int nres = pl.rb ? residue_batch_candidates(pl.rb) : 1;
-    int nslice = 0;
-    for (const Stage &st : pl.stages) nslice += st.kind == 6;
-    const int nc = nslice ? 1 + (int)pl.gemm_cands.size() : nres;
+    const int nc = pl.rb ? residue_batch_candidates(pl.rb) : 1;
     if (nc <= 1) return run_plan(pl, s);
     hipEvent_t e0, e1;
     MDG_HIP_CHECK(hipEventCreate(&e0));
@@ -710,10 +699,7 @@ static int tune_plan(Plan &pl, hipStream_t s)
     float best = 1e30f;
     int pick = 0, rc = MDG_SUCCESS;
     for (int c = 0; c < nc && rc == MDG_SUCCESS; ++c) {
-        if (nslice)
-            pl.gemm_pick = c - 1;
-        else
-            residue_batch_select(pl.rb, c);
+        residue_batch_select(pl.rb, c);
         float ms = 0.f;
         if (hipEventRecord(e0, s) != hipSuccess || (rc = run_plan(pl, s)) || hipEventRecord(e1, s) != hipSuccess ||
             hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -731,10 +717,7 @@ static int tune_plan(Plan &pl, hipStream_t s)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    if (nslice)
-        pl.gemm_pick = pick - 1;
-    else
-        residue_batch_select(pl.rb, pick);
+    residue_batch_select(pl.rb, pick);
     return MDG_SUCCESS;
 }
 
@@ -990,7 +973,7 @@ int batched_gemm(int count, const mdg_problem *pr, int method, int target_bits, 
     bool fused =
         (method == MDG_METHOD_INT8_OZAKI || method == MDG_METHOD_VECTOR || method == MDG_METHOD_INT8_RESIDUE);
     int rc = fused ? build_plan(*pl, count, pr, method, target_bits, spread) : MDG_SUCCESS;
-    if (rc == MDG_ERROR_NOT_SUPPORTED && method == MDG_METHOD_INT8_RESIDUE) {
+    if (rc == MDG_ERROR_NOT_SUPPORTED && (method == MDG_METHOD_INT8_RESIDUE || method == MDG_METHOD_INT8_OZAKI)) {
         // a problem too deep for the fused residue stages: the per-problem
         // calls inside the graph instead
         delete pl;

@@ -26,8 +26,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace mdg {
 
@@ -133,34 +136,37 @@ __host__ __device__ __forceinline__ int bnd_first_plane(int g, int Tc)
 // loads of step s+PA into the slots step s-1 freed, multiply.  A chunk that
 // ends before the band's last diagonal (t1 <= g0) pads each k-block with
 // gb-1 steps that only load B.
+template <int G, int PA>
+constexpr int bnd_lds_bytes()
+{
+    // slots: PA + 1 of the A stream, PA + G of the B stream, one block of zeros
+    return (PA + 1 + PA + G + 1) * BND_BLK;
+}
+
+// One band item of one output tile: Atile / Btile = the tile's blocked
+// slices (block (kb, slice) at (kb * S + slice) * 16 KiB), diagonals
+// g0 .. g0+G-1, slices t of A in [t0, t1); the tile's rows start at row0 of
+// the m x n planes, its columns at col0.  lds: bnd_lds_bytes<G, PA>() bytes.
 template <int G, int PA, int WM>
-__global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
-                                                      int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
-                                                      BandTable tab, int *__restrict__ D)
+__device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__ Atile,
+                                         const int8_t *__restrict__ Btile, int S, int nkb, int Tc, int g0, int t0,
+                                         int t1, long long row0, long long col0, int m, int n, int *__restrict__ D)
 {
     constexpr int RA = PA + 1, RB = PA + G;
     // WM x WN waves, each TMW x TNW of every diagonal: IM x IN MFMA tiles
     constexpr int WN = 4 / WM, TMW = BND_TM / WM, TNW = BND_TM / WN, IM = TMW / 32, IN = TNW / 32;
-    // slots: RA of the A stream, RB of the B stream, one block of zeros
-    __shared__ __attribute__((aligned(1024))) int8_t lds[(RA + RB + 1) * BND_BLK];
-    static_assert((RA + RB + 1) * BND_BLK <= 163840, "LDS ring exceeds 160 KiB");
-
-    const BandItem it = tab.item[blockIdx.y];
-    const int g0 = it.g0, t0 = it.t0, t1 = it.t1;
-    constexpr int gb = G;   // items of this launch have exactly G diagonals
+    static_assert(bnd_lds_bytes<G, PA>() <= 163840, "LDS ring exceeds 160 KiB");
+    constexpr int gb = G;   // items of one launch have exactly G diagonals
     const int nA = t1 - t0;
     const int nsk = (t1 <= g0 + 1) ? nA + gb - 1 : g0 + gb - t0;
     const int uhi = g0 + gb - 1 - t0;
     const int nsteps = nkb * nsk;
 
-    int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave / WN, wn = wave % WN;
-    const long long tstride = (long long)nkb * S * BND_BLK;
-    const int8_t *Arow = Ab + tm * tstride + wave * 4096 + lane * 16;   // per-lane base of this wave's 4 KiB
-    const int8_t *Brow = Bb + tn * tstride + wave * 4096 + lane * 16;
+    const int8_t *Arow = Atile + wave * 4096 + lane * 16;   // per-lane base of this wave's 4 KiB
+    const int8_t *Brow = Btile + wave * 4096 + lane * 16;
     int8_t *ldsA = lds + wave * 4096;
     int8_t *ldsB = lds + RA * BND_BLK + wave * 4096;
 
@@ -343,14 +349,59 @@ __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__
         for (int i = 0; i < IM; ++i)
 #pragma unroll
             for (int jj = 0; jj < IN; ++jj) {
-                const long long col = (long long)tn * BND_TM + wn * TNW + jj * 32 + (lane & 31);
+                const long long col = col0 + wn * TNW + jj * 32 + (lane & 31);
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const long long row =
-                        (long long)tm * BND_TM + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                        row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                     if (row < m && col < n) plane[row * n + col] = acc[j][i][jj][v];
                 }
             }
+    }
+}
+
+
+// Single problem.  Grid: x = output tiles (tiles_m * tiles_n), y = band items.
+template <int G, int PA, int WM>
+__global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
+                                                      int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
+                                                      BandTable tab, int *__restrict__ D)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
+    const BandItem it = tab.item[blockIdx.y];
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const long long tstride = (long long)nkb * S * BND_BLK;
+    band_run<G, PA, WM>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, Tc, it.g0, it.t0, it.t1,
+                        (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
+}
+
+// Band items: bands of G diagonals, each band's t-range cut into chunks of
+// Tc slices (work = slice-pair blocks, padding steps included).  When S is
+// not a multiple of G the narrower band (*gpart wide, in `part`) holds the
+// cheapest diagonals, 0 .. S%G - 1 (one slice pair for diagonal 0): a
+// narrow band reuses its operands less, so it should carry little work.
+// Bands of exactly G diagonals go to `full`.
+struct BandWork {
+    BandItem it;
+    long long work;
+};
+
+inline void band_items(int S, int G, int Tc, std::vector<BandWork> &full, std::vector<BandWork> &part, int *gpart)
+{
+    full.clear();
+    part.clear();
+    *gpart = S % G;
+    std::vector<std::pair<int, int>> bands;   // (first diagonal, width)
+    if (*gpart) bands.push_back({0, *gpart});
+    for (int g0 = *gpart; g0 < S; g0 += G) bands.push_back({g0, G});
+    for (const auto &bd : bands) {
+        const int g0 = bd.first, gb = bd.second;
+        for (int t0 = 0; t0 < g0 + gb; t0 += Tc) {
+            const int t1 = std::min(t0 + Tc, g0 + gb);
+            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + gb - 1 : g0 + gb - t0;
+            (gb == G ? full : part).push_back(BandWork{BandItem{(short)g0, (short)gb, (short)t0, (short)t1}, steps * gb});
+        }
     }
 }
 

@@ -781,36 +781,15 @@ BlasHandle *blas_handle()
 // the vendor libraries, kept only as comparison engines.
 enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
 
-// Band items: bands of G diagonals, each band's t-range cut into chunks of
-// Tc slices, heaviest first (work = slice-pair blocks, padding steps
-// included).  When S is not a multiple of G the narrower band (its own
-// kernel instance, *gpart wide, in `part`) holds the cheapest diagonals,
-// 0 .. S%G - 1 (one slice pair for diagonal 0): a narrow band reuses its
-// operands less, so it should carry little work.  Bands of exactly G
-// diagonals go to `full`.
+// Band tables of one problem (band_items), heaviest items first.
 static bool build_band_tables(int S, int G, int Tc, BandTable &full, BandTable &part, int *gpart)
 {
-    struct W {
-        BandItem it;
-        long long work;
-    };
-    std::vector<W> v[2];
-    *gpart = S % G;
-    std::vector<std::pair<int, int>> bands;   // (first diagonal, width)
-    if (*gpart) bands.push_back({0, *gpart});
-    for (int g0 = *gpart; g0 < S; g0 += G) bands.push_back({g0, G});
-    for (const auto &bd : bands) {
-        const int g0 = bd.first, gb = bd.second;
-        for (int t0 = 0; t0 < g0 + gb; t0 += Tc) {
-            const int t1 = std::min(t0 + Tc, g0 + gb);
-            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + gb - 1 : g0 + gb - t0;
-            v[gb == G ? 0 : 1].push_back(W{BandItem{(short)g0, (short)gb, (short)t0, (short)t1}, steps * gb});
-        }
-    }
+    std::vector<BandWork> v[2];
+    band_items(S, G, Tc, v[0], v[1], gpart);
     BandTable *tabs[2] = {&full, &part};
     for (int c = 0; c < 2; ++c) {
         if ((int)v[c].size() > BND_MAX_ITEMS) return false;
-        std::stable_sort(v[c].begin(), v[c].end(), [](const W &x, const W &y) { return x.work > y.work; });
+        std::stable_sort(v[c].begin(), v[c].end(), [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
         tabs[c]->count = (int)v[c].size();
         for (int i = 0; i < tabs[c]->count; ++i) tabs[c]->item[i] = v[c][i].it;
     }
