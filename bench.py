@@ -53,7 +53,7 @@ def engine_name(path, backend):
         return ENGINE_NAMES.get(backend, str(backend))
     if path == "int8_residue":
         return {1: "rocblas", 3: "hipblaslt"}.get(backend, str(backend))
-    return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel", "bf16": "diag_bf16_kernel",
+    return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel", "bf16": "band_bf16_kernel",
             "vector": "vector_gemm_kernel"}.get(path, path)
 
 
@@ -227,8 +227,8 @@ def bench_batched(a):
     head = results["int8"]
     achieved = w_mfma / (head["gemm_stage_ms"] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                "frac": achieved / INT8_PEAK_TOPS, "traffic": read_traffic("b_diag_i8_kernel", "cfg4"),
-                "kernel": "b_diag_i8_kernel", "launches_per_step": 1, "work_per_launch": w_mfma,
+                "frac": achieved / INT8_PEAK_TOPS, "traffic": read_traffic("b_band_i8_kernel", "cfg4"),
+                "kernel": "b_band_i8_kernel", "launches_per_step": 1, "work_per_launch": w_mfma,
                 "avg_launch_ms": head["gemm_stage_ms"],
                 "note": "achieved = sum over problems of S(S+1)/2 * 2mnk INT8 ops / GEMM-stage time from HIP "
                         "events recorded into the replayed graph"}
@@ -368,7 +368,7 @@ def main():
     elif head_path == "bf16":
         work = P * 2.0 * m * n * k
         peak, unit = BF16_PEAK_TFLOPS, "TFLOP/s"
-        kernel = "diag_bf16_kernel"
+        kernel = head["engine"]
     else:
         work = P * 2.0 * m * n * k
         peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
@@ -378,6 +378,9 @@ def main():
     if head_path == "int8" and head["engine"] == "band_i8_kernel":
         # one launch over the bands of 4 diagonals, one over the narrower last band
         launches = 1 + (1 if head["slices"] % 4 else 0)
+    elif head_path == "bf16" and head["engine"] == "band_bf16_kernel":
+        # bands of 3 diagonals, and a narrower band for the remainder
+        launches = 1 + (1 if head["slices"] % 3 else 0)
     elif head_path == "int8" and head["engine"] in ("rocblas", "hipblaslt"):
         kp = -(-k // 128) * 128
         kmax = (131071 // kp) * kp if kp <= 131071 else 130944

@@ -68,6 +68,15 @@ struct BandTable {
 
 typedef int bnd_i4 __attribute__((ext_vector_type(4)));
 typedef int bnd_i16 __attribute__((ext_vector_type(16)));
+typedef float bnd_f16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bnd_bf8 __attribute__((ext_vector_type(8)));
+
+// f(integral_constant<int, R>) for R in the sequence, in order
+template <int... R, typename F>
+__device__ __forceinline__ void bnd_unroll(std::integer_sequence<int, R...>, F &&f)
+{
+    (f(std::integral_constant<int, R>{}), ...);
+}
 
 // s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding: vmcnt[3:0] in bits 3:0,
 // vmcnt[5:4] in bits 15:14, expcnt left at its maximum, lgkmcnt 0): every
@@ -147,7 +156,7 @@ constexpr int bnd_lds_bytes()
 // slices (block (kb, slice) at (kb * S + slice) * 16 KiB), diagonals
 // g0 .. g0+G-1, slices t of A in [t0, t1); the tile's rows start at row0 of
 // the m x n planes, its columns at col0.  lds: bnd_lds_bytes<G, PA>() bytes.
-template <int G, int PA, int WM>
+template <int G, int PA, int WM, bool BF = false>
 __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__ Atile,
                                          const int8_t *__restrict__ Btile, int S, int nkb, int Tc, int g0, int t0,
                                          int t1, long long row0, long long col0, int m, int n, int *__restrict__ D)
@@ -228,7 +237,29 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         for (int i = 0; i < BND_BLK / (16 * 256); ++i) z[threadIdx.x + 256 * i] = (bnd_i4){0, 0, 0, 0};
     }
 
-    bnd_i16 acc[G][IM][IN];
+    // BF (BF16 slices): v_mfma_f32_32x32x16_bf16 reads its two 16-byte
+    // fragments with the byte layout of v_mfma_i32_32x32x32_i8 (8 bf16 of a
+    // 16-element k-step per lane), so blocks, ring and fragment reads are the
+    // same; the FP32 sums of each tile are moved into INT32 sums (tot) while
+    // they are still exact integers
+    using AccT = std::conditional_t<BF, bnd_f16, bnd_i16>;
+    AccT acc[G][IM][IN];
+    bnd_i16 tot[BF ? G : 1][IM][IN];
+    if constexpr (BF) {
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
+#pragma unroll
+                for (int jj = 0; jj < IN; ++jj)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) tot[j][i][jj][v] = 0;
+    }
+    // BF16 tiles are moved into tot KT at a time, one group per step
+    // (groups of KT consecutive tiles, NB groups)
+    constexpr int NT = G * IM * IN;
+    constexpr int KT = (NT + 7) / 8, NB = NT / KT;
+    static_assert(NB * KT == NT, "tile groups must cover the tiles");
 #pragma unroll
     for (int j = 0; j < G; ++j)
 #pragma unroll
@@ -236,7 +267,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 #pragma unroll
             for (int jj = 0; jj < IN; ++jj)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) acc[j][i][jj][v] = 0;
+                for (int v = 0; v < 16; ++v) acc[j][i][jj][v] = 0;   // 0 or 0.f
 
     // step state: q = index of step s within its k-block, sa = its A slot,
     // sb = the B slot of its window position s + gb - 1 (diagonal j = 0);
@@ -269,15 +300,25 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 #pragma unroll
             for (int jj = 0; jj < IN; ++jj) b[buf][j][jj] = *(const bnd_i4 *)(pb[j] + jj * 4096 + koff[ks]);
     };
-    auto mfma = [&](int buf) {
+    // MFMAs of one k-step over the NT accumulator tiles (tile T = (j IM + i)
+    // IN + jj) in the order O, O+1, ... mod NT; the tiles of group ZT >= 0
+    // (BF16 only) start from 0: their sums were copied out in the step before
+    auto mfma = [&](int buf, auto oc, auto zc) {
+        constexpr int O = decltype(oc)::value, ZT = decltype(zc)::value;
 #pragma unroll
-        for (int j = 0; j < G; ++j)
-#pragma unroll
-            for (int i = 0; i < IM; ++i)
-#pragma unroll
-                for (int jj = 0; jj < IN; ++jj)
-                    acc[j][i][jj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[buf][i], b[buf][j][jj], acc[j][i][jj], 0, 0, 0);
+        for (int idx = 0; idx < NT; ++idx) {
+            const int T = (O + idx) % NT;
+            const int j = T / (IM * IN), i = (T / IN) % IM, jj = T % IN;
+            if constexpr (BF)
+                acc[j][i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bnd_bf8, a[buf][i]), __builtin_bit_cast(bnd_bf8, b[buf][j][jj]),
+                    (ZT >= 0 && T / KT == ZT) ? (bnd_f16){} : acc[j][i][jj], 0, 0, 0);
+            else
+                acc[j][i][jj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[buf][i], b[buf][j][jj], acc[j][i][jj], 0, 0, 0);
+        }
     };
+    using C0 = std::integral_constant<int, 0>;
+    using NoZero = std::integral_constant<int, -1>;
 
     // The barrier of step s sits inside its last k-step: wait for this
     // wave's loads of step s+1 (and for its own fragment reads), barrier,
@@ -297,11 +338,20 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     // after each MFMA, so no burst of reads or copies leaves the MFMA pipe
     // idle: nr = fragment reads per k-step, 4 * G * ... MFMAs per k-step
     constexpr int NR = IM + IN * G, NM = IM * IN * G;
-    for (int s = 0; s < nsteps; ++s) {
+    // BF16 moves, a group of tiles at a time: a step with NX >= 0 runs group
+    // NX first in its last k-step and adds its sums into tot under the other
+    // tiles' MFMAs; a step with FT >= 0 restarts group FT (added into tot at
+    // the end of the step before) from 0 in its first k-step, where those
+    // tiles go last
+    auto step = [&](auto fc, auto nc) {
+        constexpr int FT = decltype(fc)::value, NX = decltype(nc)::value;
 #pragma unroll
         for (int ks = 0; ks < 3; ++ks) {
             fetch((ks & 1) ^ 1, ks + 1);
-            mfma(ks & 1);
+            if (ks == 0)
+                mfma(0, std::integral_constant<int, ((FT + 1) * KT) % NT>{}, fc);
+            else
+                mfma(ks & 1, C0{}, NoZero{});
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
@@ -320,18 +370,67 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         fetch(0, 0);
         issue_a();
         issue_b();
-        mfma(1);
+        mfma(1, std::integral_constant<int, (NX > 0 ? NX * KT : 0)>{}, NoZero{});
+        if constexpr (BF && NX >= 0) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
-        }
+            for (int e = 0; e < KT; ++e) {
+                const int T = NX * KT + e;
+                const int j = T / (IM * IN), i = (T / IN) % IM, jj = T % IN;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global->LDS copy
+                for (int v = 0; v < 16; ++v) tot[j][i][jj][v] += (int)acc[j][i][jj][v];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
+            }
+            // after the group's own MFMAs (KT <= NR): MFMA, copy and a share
+            // of the 48 KT moves (read, convert, add)
+            constexpr int NV = (48 * KT + 7) / 8;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global->LDS copy
+                __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);  // VALU
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global->LDS copy
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
+    };
+    bool drop = false;   // BF16: group NB-1's acc is already in tot
+    if constexpr (BF) {
+        // FP32 sums stay exact below 2^24, i.e. for 16 steps (1024 products
+        // of at most 2^14).  In blocks of NB steps, step r restarts group
+        // (r - 1) mod NB and adds group r into tot, so each group is added
+        // and restarted once per NB steps (the first restart of group NB-1
+        // finds 0).  After the last block group NB-1 is in tot but not
+        // restarted: the first of the other nsteps % NB steps restarts it,
+        // and with none left the epilogue leaves its acc out (drop).  Group
+        // r < NB-1 then holds NB-1-r + nsteps % NB <= 2 NB - 2 steps.
+        static_assert(2 * NB - 2 <= 16, "tiles must be restarted within 16 steps");
+        int s = 0;
+        for (; s + NB <= nsteps; s += NB)
+            bnd_unroll(std::make_integer_sequence<int, NB>{}, [&](auto rc) {
+                constexpr int R = decltype(rc)::value;
+                step(std::integral_constant<int, (R + NB - 1) % NB>{}, rc);
+            });
+        drop = s > 0 && s == nsteps;
+        if (s < nsteps) {
+            step(std::integral_constant<int, NB - 1>{}, NoZero{});
+            for (++s; s < nsteps; ++s) step(NoZero{}, NoZero{});
+        }
+    } else {
+        for (int s = 0; s < nsteps; ++s) step(NoZero{}, NoZero{});
     }
     // no global->LDS copy may still be in flight when the workgroup ends
     bnd_wait_vm<0>();
@@ -354,7 +453,14 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
                 for (int v = 0; v < 16; ++v) {
                     const long long row =
                         row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                    if (row < m && col < n) plane[row * n + col] = acc[j][i][jj][v];
+                    if (row < m && col < n) {
+                        if constexpr (BF)
+                            plane[row * n + col] =
+                                tot[j][i][jj][v] +
+                                ((drop && ((j * IM + i) * IN + jj) / KT == NB - 1) ? 0 : (int)acc[j][i][jj][v]);
+                        else
+                            plane[row * n + col] = acc[j][i][jj][v];
+                    }
                 }
             }
     }
@@ -374,6 +480,21 @@ __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__
     const long long tstride = (long long)nkb * S * BND_BLK;
     band_run<G, PA, WM>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, Tc, it.g0, it.t0, it.t1,
                         (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
+}
+
+// The same over BF16 slices (64 bf16 per block row, nkb = kp / 64).
+template <int G, int PA, int WM>
+__global__ __launch_bounds__(256) void band_bf16_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
+                                                        int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
+                                                        BandTable tab, int *__restrict__ D)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
+    const BandItem it = tab.item[blockIdx.y];
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const long long tstride = (long long)nkb * S * BND_BLK;
+    band_run<G, PA, WM, true>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, Tc, it.g0, it.t0, it.t1,
+                              (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
 }
 
 // Band items: bands of G diagonals, each band's t-range cut into chunks of

@@ -245,23 +245,27 @@ __global__ __launch_bounds__(256) void slice_b_kernel(int k, int n, int kp, int 
     fixed_digits<NW>(N, S, w, [&](int u, int d) { o[(long long)(S - 1 - u) * kp] = digit_as<T>(d); });
 }
 
-// INT8 slices in the blocked layout of the band kernel (i8band.hpp): thread
-// per entry (r, kk) of the padded rp x kp domain, kk fastest; slice t of the
-// entry lands in block (r / 128, kk / 128, t) at bnd_pos(r % 128, kk % 128).
-// TRANS: the operand is B (k x n, row stride ldx) and "row" r is column r of B.
-template <int L, bool TRANS>
+// INT8 (or BF16) slices in the blocked layout of the band kernel
+// (i8band.hpp): thread per entry (r, kk) of the padded rp x kp domain, kk
+// fastest; a block row holds EB = 128 / sizeof(T) entries, so slice t of the
+// entry lands in block (r / 128, kk / EB, t) at byte bnd_pos(r % 128,
+// sizeof(T) (kk % EB)).  TRANS: the operand is B (k x n, row stride ldx) and
+// "row" r is column r of B.
+template <int L, bool TRANS, typename T = int8_t>
 __global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int rp, int kp, const double *__restrict__ X,
                                                         long long ldx, const int *__restrict__ E, int S, int w,
                                                         int8_t *__restrict__ out)
 {
     constexpr int NW = FixedWords<L>::NW;
+    constexpr int EB = BND_BK / (int)sizeof(T);
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)rp * kp) return;
     const int r = (int)(idx / kp);
     const int kk = (int)(idx - (long long)r * kp);
-    int8_t *o = out + bnd_block(r >> 7, kk >> 7, 0, S, kp >> 7) + bnd_pos(r & 127, kk & 127);
+    T *o = (T *)(out + bnd_block(r >> 7, kk / EB, 0, S, kp / EB) + bnd_pos(r & 127, (int)sizeof(T) * (kk % EB)));
+    constexpr long long TS = BND_BLK / (int)sizeof(T);   // slice stride in T
     if (r >= rows || kk >= k || E[r] == ZERO_SCALE) {
-        for (int t = 0; t < S; ++t) o[(long long)t * BND_BLK] = 0;
+        for (int t = 0; t < S; ++t) o[(long long)t * TS] = digit_as<T>(0);
         return;
     }
     double x[L];
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int rp,
         x[l] = TRANS ? X[((long long)kk * ldx + r) * L + l] : X[((long long)r * ldx + kk) * L + l];
     uint32_t N[NW];
     to_fixed<L, NW>(x, S * w - 2 - E[r], N);
-    fixed_digits<NW>(N, S, w, [&](int t, int d) { o[(long long)t * BND_BLK] = (int8_t)d; });
+    fixed_digits<NW>(N, S, w, [&](int t, int d) { o[(long long)t * TS] = digit_as<T>(d); });
 }
 
 // --------------------------------------------------- INT8 diagonal GEMM --
@@ -813,6 +817,22 @@ static int launch_band(int gb, const BandTable &tab, const int8_t *Aall, const i
     return MDG_SUCCESS;
 }
 
+// the same over blocked BF16 slices (nkb = kp / 64)
+static int launch_band_bf16(int gb, const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb,
+                            int Tc, int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
+{
+    if (tab.count == 0) return MDG_SUCCESS;
+    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
+    switch (gb) {
+    case 3: hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 2: hipLaunchKernelGGL((band_bf16_kernel<2, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    case 1: hipLaunchKernelGGL((band_bf16_kernel<1, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
+    default: set_error("band width %d", gb); return MDG_ERROR_INVALID_VALUE;
+    }
+    MDG_LAUNCH_CHECK("band_bf16_kernel");
+    return MDG_SUCCESS;
+}
+
 // every K-chunk plane as one INT8 GEMM on the library engines (rocBLAS or
 // hipBLASLt, autotuned per plane shape); *engine: the engine of the last plane
 static int i8_planes_library(const int8_t *Aall, const int8_t *Ball, int S, int kp, int kmax, int m, int n, int *D,
@@ -1074,7 +1094,50 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                                    0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
         });
         if (rc) return rc;
+    } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071) {
+        // band kernel over blocked BF16 slices: the INT32 chunk planes of the
+        // INT8 path (same integer sums, FP32 partial sums moved into INT32
+        // every 1024 products), so the INT8 reconstruction applies unchanged
+        const int kmax = (131071 / kp) * kp;
+        int nplanes = 0;
+        for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
+        p.wide = nplanes > S;
+        p.flush_blocks = kmax;
+        // bands of 3 diagonals: 192 FP32 sums and 192 INT32 totals per lane
+        constexpr int BAND_G = 3;
+        BandTable tab_full, tab_part;
+        int gpart = 0;
+        const bool fits = build_band_tables(S, BAND_G, kmax / kp, tab_full, tab_part, &gpart);
+        MDG_REQUIRE(fits, MDG_ERROR_RANGE, "too many BF16 band items for %d slices", S);
+        int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
+        const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
+        int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
+        int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp * 2, 3);
+        MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+        hipLaunchKernelGGL((slice_blk_kernel<L, false, __bf16>), dim3(blocks_for<int>((long long)mpb * kp, 256)),
+                           dim3(256), 0, s, m, k, mpb, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_blk_kernel<L, true, __bf16>), dim3(blocks_for<int>((long long)npb * kp, 256)),
+                           dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
+        tm.mark(2);
+        const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
+        rc = launch_band_bf16(BAND_G, tab_full, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+        if (!rc)
+            rc = launch_band_bf16(gpart, tab_part, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+        if (rc) return rc;
+        p.backend = I8E_BAND;
+        tm.mark(3);
+        rc = with_diag_words<L>(S, w, [&](auto nw) {
+            constexpr int NW = decltype(nw)::value;
+            if (nplanes > S)
+                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, true>), dim3(blocks_for<int>(mn, 256)), dim3(256),
+                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+            else
+                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, false>), dim3(blocks_for<int>(mn, 256)), dim3(256),
+                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+        });
+        if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
+        // depth beyond one INT32 chunk slice: INT64 diagonal planes
         p.wide = true;
         p.flush_blocks = 16;
         __bf16 *Aall = (__bf16 *)scratch(s, (size_t)mp * S * kp * 2, 2);

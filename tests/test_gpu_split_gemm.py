@@ -179,10 +179,43 @@ def test_int8_band_kernel_shapes(mdg, orc, torch_cuda, L, m, n, k, emax):
     dA, dB = to_dev(A), to_dev(B)
     C2, i2 = gemm(mdg, L, m, n, k, dA, dB, "int8")
     C4, i4 = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256")
+    C5, i5 = gemm(mdg, L, m, n, k, dA, dB, "bf16")
     assert i2["backend"] == 2 and i4["backend"] == 4, (i2, i4)
     assert np.array_equal(to_host(C2).view(np.uint64), to_host(C4).view(np.uint64))
+    # the BF16 band kernel sums the same slice products exactly
+    assert i5["slices"] == i2["slices"], (i5, i2)
+    assert np.array_equal(to_host(C2).view(np.uint64), to_host(C5).view(np.uint64))
     err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C2), nsamp=256)
     assert err <= _bound(L, k), (err, i2)
+
+
+def _digits_127(L, rows, cols):
+    # every entry 1/2 + sum_{t=1}^{6L-1} 127 * 2^(-8t-6), limb l holding the
+    # terms t = 6l .. 6l+5 (at most 48 bits, exact): the row scale is 2^0,
+    # so slice t >= 1 of every entry is the digit 127 and the top slice 32
+    x = np.zeros(L)
+    for t in range(1, 6 * L):
+        x[t // 6] += 127.0 * 2.0 ** (-8 * t - 6)
+    x[0] += 0.5
+    return np.tile(x, rows * cols).reshape(rows * cols * L).copy()
+
+
+@pytest.mark.parametrize("method", ["bf16", "int8"])
+def test_band_kernel_largest_partial_sums(mdg, orc, torch_cuda, method):
+    # every product of two slices is 127^2 and they all add up, so the BF16
+    # kernel's FP32 sums grow as fast as they can between moves into INT32
+    # (at most 14 steps of 64 products: 896 * 127^2 < 2^24; a sum held past
+    # 1040 products, 1041 * 127^2 odd and above 2^24, would round); the
+    # result must match the INT8 path bit for bit and the exact product
+    L, m, n, k = 4, 64, 96, 4096
+    A = _digits_127(L, m, k)
+    B = _digits_127(L, k, n)
+    dA, dB = to_dev(A), to_dev(B)
+    C, info = gemm(mdg, L, m, n, k, dA, dB, method)
+    C8, _ = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256")
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C8).view(np.uint64)), info
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=128)
+    assert err <= _bound(L, k), (err, info)
 
 
 def test_int8_deep_k_falls_back_to_tile256(mdg, orc, torch_cuda):
@@ -195,6 +228,9 @@ def test_int8_deep_k_falls_back_to_tile256(mdg, orc, torch_cuda):
     assert info["backend"] == 4, info
     err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=96)
     assert err <= _bound(L, k), (err, info)
+    # BF16 at this depth: INT64 diagonal planes instead of INT32 chunks
+    C2, info2 = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), "bf16")
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C2).view(np.uint64)), info2
 
 
 def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):
