@@ -259,7 +259,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     // (groups of KT consecutive tiles, NB groups)
     constexpr int NT = G * IM * IN;
     constexpr int KT = (NT + 7) / 8, NB = NT / KT;
-    static_assert(NB * KT == NT, "tile groups must cover the tiles");
+    static_assert(!BF || NB * KT == NT, "tile groups must cover the tiles");
 #pragma unroll
     for (int j = 0; j < G; ++j)
 #pragma unroll
@@ -292,23 +292,28 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         }
     };
     bnd_i4 a[2][IM], b[2][G][IN];
-    auto fetch = [&](int buf, int ks) {
+    // fragments of k-step ks; diagonals j < J0 are not read (no MFMA uses them)
+    auto fetch = [&](int buf, int ks, auto jc) {
+        constexpr int J0 = decltype(jc)::value;
 #pragma unroll
         for (int i = 0; i < IM; ++i) a[buf][i] = *(const bnd_i4 *)(pa + i * 4096 + koff[ks]);
 #pragma unroll
-        for (int j = 0; j < G; ++j)
+        for (int j = J0; j < G; ++j)
 #pragma unroll
             for (int jj = 0; jj < IN; ++jj) b[buf][j][jj] = *(const bnd_i4 *)(pb[j] + jj * 4096 + koff[ks]);
     };
+    using J0All = std::integral_constant<int, 0>;
     // MFMAs of one k-step over the NT accumulator tiles (tile T = (j IM + i)
-    // IN + jj) in the order O, O+1, ... mod NT; the tiles of group ZT >= 0
-    // (BF16 only) start from 0: their sums were copied out in the step before
-    auto mfma = [&](int buf, auto oc, auto zc) {
-        constexpr int O = decltype(oc)::value, ZT = decltype(zc)::value;
+    // IN + jj) in the order O, O+1, ... mod NT, diagonals j >= J0 only; the
+    // tiles of group ZT >= 0 (BF16 only) start from 0: their sums were
+    // copied out in the step before
+    auto mfma = [&](int buf, auto oc, auto zc, auto jc) {
+        constexpr int O = decltype(oc)::value, ZT = decltype(zc)::value, J0 = decltype(jc)::value;
 #pragma unroll
         for (int idx = 0; idx < NT; ++idx) {
             const int T = (O + idx) % NT;
             const int j = T / (IM * IN), i = (T / IN) % IM, jj = T % IN;
+            if (j < J0) continue;
             if constexpr (BF)
                 acc[j][i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                     __builtin_bit_cast(bnd_bf8, a[buf][i]), __builtin_bit_cast(bnd_bf8, b[buf][j][jj]),
@@ -333,31 +338,33 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     issue_a();
     issue_b();
     set_step();
-    fetch(0, 0);
-    // the scheduler places one fragment read (or one global->LDS copy)
-    // after each MFMA, so no burst of reads or copies leaves the MFMA pipe
-    // idle: nr = fragment reads per k-step, 4 * G * ... MFMAs per k-step
-    constexpr int NR = IM + IN * G, NM = IM * IN * G;
+    fetch(0, 0, J0All{});
     // BF16 moves, a group of tiles at a time: a step with NX >= 0 runs group
     // NX first in its last k-step and adds its sums into tot under the other
     // tiles' MFMAs; a step with FT >= 0 restarts group FT (added into tot at
     // the end of the step before) from 0 in its first k-step, where those
-    // tiles go last
-    auto step = [&](auto fc, auto nc) {
-        constexpr int FT = decltype(fc)::value, NX = decltype(nc)::value;
+    // tiles go last.  J0: the step's first diagonal with a slice pair (the
+    // MFMAs of diagonals j < J0 would only add zero products and are left
+    // out; J0 = G: a padding step, copies and barrier only)
+    auto step = [&](auto fc, auto nc, auto jc) {
+        constexpr int FT = decltype(fc)::value, NX = decltype(nc)::value, J0 = decltype(jc)::value;
+        // the scheduler places one fragment read (or one global->LDS copy)
+        // after each MFMA, so no burst of reads or copies leaves the MFMA
+        // pipe idle: NR = fragment reads, NM = MFMAs per k-step
+        constexpr int NR = IM + IN * (G - J0), NM = IM * IN * (G - J0), NR0 = IM + IN * G;
 #pragma unroll
         for (int ks = 0; ks < 3; ++ks) {
-            fetch((ks & 1) ^ 1, ks + 1);
+            fetch((ks & 1) ^ 1, ks + 1, jc);
             if (ks == 0)
-                mfma(0, std::integral_constant<int, ((FT + 1) * KT) % NT>{}, fc);
+                mfma(0, std::integral_constant<int, ((FT + 1) * KT) % NT>{}, fc, jc);
             else
-                mfma(ks & 1, C0{}, NoZero{});
+                mfma(ks & 1, C0{}, NoZero{}, jc);
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
+            for (int r = 0; r < (NR < NM ? NR : NM); ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
             }
-            __builtin_amdgcn_sched_group_barrier(0x008, NM - NR, 0);
+            if constexpr (NM > NR) __builtin_amdgcn_sched_group_barrier(0x008, NM - NR, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
         bnd_wait_vm<8 * (PA - 1)>();
@@ -367,10 +374,12 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         if (++sa == RA) sa = 0;
         if (++sb == RB) sb = 0;
         set_step();
-        fetch(0, 0);
+        fetch(0, 0, J0All{});
         issue_a();
         issue_b();
-        mfma(1, std::integral_constant<int, (NX > 0 ? NX * KT : 0)>{}, NoZero{});
+        mfma(1, std::integral_constant<int, (NX > 0 ? NX * KT : 0)>{}, NoZero{}, jc);
+        constexpr int NR1 = NR0 < NM ? NR0 : NM;
+        constexpr int NC = (NM - NR1) < 8 ? (NM - NR1) : 8;
         if constexpr (BF && NX >= 0) {
 #pragma unroll
             for (int e = 0; e < KT; ++e) {
@@ -380,7 +389,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
                 for (int v = 0; v < 16; ++v) tot[j][i][jj][v] += (int)acc[j][i][jj][v];
             }
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
+            for (int r = 0; r < NR1; ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
             }
@@ -395,12 +404,12 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
             }
         } else {
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
+            for (int r = 0; r < NR1; ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
             }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
+            for (int r = 0; r < NC; ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
                 __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // global->LDS copy
             }
@@ -422,15 +431,28 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         for (; s + NB <= nsteps; s += NB)
             bnd_unroll(std::make_integer_sequence<int, NB>{}, [&](auto rc) {
                 constexpr int R = decltype(rc)::value;
-                step(std::integral_constant<int, (R + NB - 1) % NB>{}, rc);
+                step(std::integral_constant<int, (R + NB - 1) % NB>{}, rc, J0All{});
             });
         drop = s > 0 && s == nsteps;
         if (s < nsteps) {
-            step(std::integral_constant<int, NB - 1>{}, NoZero{});
-            for (++s; s < nsteps; ++s) step(NoZero{}, NoZero{});
+            step(std::integral_constant<int, NB - 1>{}, NoZero{}, J0All{});
+            for (++s; s < nsteps; ++s) step(NoZero{}, NoZero{}, J0All{});
         }
     } else {
-        for (int s = 0; s < nsteps; ++s) step(NoZero{}, NoZero{});
+        // per k-block: nfull steps with a slice pair on every diagonal, the
+        // steps whose first J0 = jlo .. jhi diagonals have none (slice t of
+        // A above g0), and npad padding steps, which only refill the B window
+        const int nfull = max(0, min(nA, g0 + 1 - t0));
+        const int jlo = max(1, t0 + nfull - g0), jhi = min(G - 1, t1 - 1 - g0);
+        const int npad = nsk - nA;
+        for (int kb = 0; kb < nkb; ++kb) {
+            for (int c = 0; c < nfull; ++c) step(NoZero{}, NoZero{}, J0All{});
+            bnd_unroll(std::make_integer_sequence<int, G - 1>{}, [&](auto rc) {
+                constexpr int J = decltype(rc)::value + 1;
+                if (J >= jlo && J <= jhi) step(NoZero{}, NoZero{}, std::integral_constant<int, J>{});
+            });
+            for (int c = 0; c < npad; ++c) step(NoZero{}, NoZero{}, std::integral_constant<int, G>{});
+        }
     }
     // no global->LDS copy may still be in flight when the workgroup ends
     bnd_wait_vm<0>();
