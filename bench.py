@@ -40,8 +40,10 @@ BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 # C++/hexa_double_functions.cu)
 VECTOR_OPS_PER_MAC = {2: 43, 4: 425, 8: 2011, 16: 11800}
 LIMB_NAMES = {2: "double-double", 4: "quad-double", 8: "octo-double", 16: "hexa-double"}
-# INT8 slice-plane engines (mdg_set_int8_backend) and their names in the output
+# INT8 engines (mdg_set_int8_backend) of the slice and residue paths, and
+# their names in the output
 ENGINE_OF_PATH = {"int8": 0, "int8_tensile": 3, "int8_tile256": 4}
+RESIDUE_ENGINE_OF_PATH = {"int8_residue": 0, "int8_residue_tensile": 3}
 ENGINE_NAMES = {1: "rocblas", 2: "band_i8_kernel", 3: "hipblaslt", 4: "diag_i8_kernel"}
 
 
@@ -51,8 +53,8 @@ def engine_name(path, backend):
     on a library; FP64 and BF16 products on this library's MFMA kernels."""
     if path in ENGINE_OF_PATH:
         return ENGINE_NAMES.get(backend, str(backend))
-    if path == "int8_residue":
-        return {1: "rocblas", 3: "hipblaslt"}.get(backend, str(backend))
+    if path in RESIDUE_ENGINE_OF_PATH:
+        return {1: "rocblas", 2: "plain_i8_kernel", 3: "hipblaslt"}.get(backend, str(backend))
     return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel", "bf16": "band_bf16_kernel",
             "vector": "vector_gemm_kernel"}.get(path, path)
 
@@ -70,13 +72,17 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--skip-vector", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--paths", default="int8,int8_tensile,int8_tile256,int8_residue,f64_residue,f64,bf16,vector",
+    ap.add_argument("--paths",
+                    default="int8,int8_tensile,int8_tile256,int8_residue,int8_residue_tensile,f64_residue,f64,"
+                            "bf16,vector",
                     help="int8 = INT8 slices on this library's band kernel (the default engine), "
                          "int8_tensile = the same slices on hipBLASLt's Tensile INT8 GEMMs (comparison "
                          "only), int8_tile256 = the same on this library's 256 x 256 single-diagonal "
-                         "kernel, int8_residue = INT8 residues modulo coprime moduli (CRT), f64_residue = "
-                         "the same on FP64 MFMA with 21-bit primes, f64 = FP64-MFMA split, bf16 = BF16 "
-                         "slices, vector = one thread per output entry")
+                         "kernel, int8_residue = INT8 residues modulo coprime moduli (CRT) on this "
+                         "library's plain INT8 GEMM kernel, int8_residue_tensile = the same residues on "
+                         "hipBLASLt (comparison only), f64_residue = the same on FP64 MFMA with 21-bit "
+                         "primes, f64 = FP64-MFMA split, bf16 = BF16 slices, vector = one thread per "
+                         "output entry")
     ap.add_argument("--elementwise", action="store_true",
                     help="time the elementwise operations instead of the GEMM")
     ap.add_argument("--ew-entries", type=int, default=1 << 22)
@@ -306,15 +312,15 @@ def main():
     outputs = {}
     mdg.set_profiling(True)
     for path in paths:
-        steps = a.steps if path in ("int8", "int8_residue") else (1 if path == "vector" else max(1, a.steps // 4))
+        steps = a.steps if path in ("int8", "int8_residue", "int8_residue_tensile") else (1 if path == "vector" else max(1, a.steps // 4))
         warm = a.warmup if path != "vector" else 0
         if path == "vector":
             # load the kernel on a small problem first
             s = mdg.fill_random(mdg.GEN_QD, L, 64, 64, 3, -8, 8)
             mdg.gemm(L, 64, 64, 64, s, s, method="vector")
         log(f"{path}: warmup {warm}, steps {steps}")
-        method = "int8" if path in ENGINE_OF_PATH else path
-        mdg.set_int8_backend(ENGINE_OF_PATH.get(path, 0))
+        method = "int8" if path in ENGINE_OF_PATH else ("int8_residue" if path in RESIDUE_ENGINE_OF_PATH else path)
+        mdg.set_int8_backend(ENGINE_OF_PATH.get(path, RESIDUE_ENGINE_OF_PATH.get(path, 0)))
         for _ in range(warm):
             mdg.gemm(L, m, n, k, A, B, method=method, C=C)
         torch.cuda.synchronize()
@@ -361,7 +367,7 @@ def main():
     head_path = "int8" if "int8" in results else next(iter(results))
     P = head["pair_gemms"]
     gemm_ms = head["stage_ms"][2]
-    if head_path == "int8":
+    if head_path == "int8" or head_path in RESIDUE_ENGINE_OF_PATH:
         work = P * 2.0 * m * n * k
         peak, unit = INT8_PEAK_TOPS, "TFLOP/s"
         kernel = head["engine"]
@@ -462,7 +468,8 @@ def main():
               "int8_residue_over_int8": ratio("int8_residue", "int8"),
               "int8_residue_over_f64_residue": ratio("int8_residue", "f64_residue"),
               "f64_residue_over_vector": ratio("f64_residue", "vector"),
-              "int8_over_int8_tensile": ratio("int8", "int8_tensile")}
+              "int8_over_int8_tensile": ratio("int8", "int8_tensile"),
+              "int8_residue_over_int8_residue_tensile": ratio("int8_residue", "int8_residue_tensile")}
     # the two speedup targets, with the pair of paths that meets each: INT8
     # over FP64-MFMA >= 3, FP64-MFMA over the vector GEMM >= 5; slices and
     # residues are both exact fixed-point products of the same inputs

@@ -1,5 +1,7 @@
-// i8gemm.hip -- INT8 x INT8 -> INT32 strided-batched GEMMs on the library
-// engines (rocBLAS, hipBLASLt), autotuned per shape.
+// i8gemm.hip -- INT8 x INT8 -> INT32 strided-batched GEMMs: this library's
+// MFMA kernel (plain_i8_kernel, i8plain.hpp) by default; the library engines
+// (rocBLAS, hipBLASLt, autotuned per shape) when mdg_set_int8_backend picks
+// them, as a comparison, or for a shape the kernel does not take.
 //
 // Layout (column-major view, the convention of both libraries): for batch b
 //   D_b (n x m, leading dimension ldd) = Bt_b^T (n x depth) * A_b (depth x m)
@@ -20,9 +22,32 @@
 #include <tuple>
 #include <vector>
 
+#include "i8plain.hpp"
 #include "split_common.hpp"
 
 namespace mdg {
+
+// the hand-written engine takes depths in whole 128-byte k-blocks and
+// 16-byte aligned operand rows
+static bool plain_fits(int batch, int depth, const int8_t *Bt, long long ldb, long long sb, const int8_t *A,
+                       long long lda, long long sa)
+{
+    auto al16 = [](long long v) { return (v & 15) == 0; };
+    return batch <= 65535 && depth > 0 && depth % PLN_BK == 0 && al16(ldb) && al16(lda) && al16(sb) && al16(sa) &&
+           al16((long long)(uintptr_t)Bt) && al16((long long)(uintptr_t)A);
+}
+
+static int run_plain(int batch, int m, int n, int depth, const int8_t *Bt, long long ldb, long long sb,
+                     const int8_t *A, long long lda, long long sa, int *D, long long ldd, long long sd, hipStream_t s)
+{
+    const int tiles_m = (m + PLN_TM - 1) / PLN_TM, tiles_n = (n + PLN_TM - 1) / PLN_TM;
+    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)batch);
+    // rows past m (n) of a 256-row tile reread row m-1 (n-1): always readable
+    hipLaunchKernelGGL(plain_i8_kernel<0>, grid, dim3(256), 0, s, A, lda, sa, m, Bt, ldb, sb, n, m, n, depth, D, ldd,
+                       sd, tiles_m, tiles_n);
+    MDG_LAUNCH_CHECK("plain_i8_kernel");
+    return MDG_SUCCESS;
+}
 
 namespace {
 
@@ -112,10 +137,15 @@ int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long l
                long long lda, long long sa, int *D, long long ldd, long long sd, hipStream_t s, int *engine_used)
 {
     if (batch <= 0 || m <= 0 || n <= 0) return MDG_SUCCESS;
+    // 1 rocBLAS, 3 hipBLASLt (comparison engines); otherwise the MFMA kernel
+    const int forced = int8_backend();
+    if (forced != 1 && forced != 3 && plain_fits(batch, depth, Bt, ldb, sb, A, lda, sa)) {
+        if (engine_used) *engine_used = 2;
+        return run_plain(batch, m, n, depth, Bt, ldb, sb, A, lda, sa, D, ldd, sd, s);
+    }
     const int32_t alpha = 1, beta = 0;
     int dev = 0;
     MDG_HIP_CHECK(hipGetDevice(&dev));
-    const int forced = int8_backend();   // 0 autotuned, 1 rocBLAS, 3 hipBLASLt
     const ShapeKey key{dev, forced, lg2_ceil(batch), lg2_ceil(m), lg2_ceil(n), lg2_ceil(depth)};
 
     BlasHandle *bh = blas_handle();

@@ -256,20 +256,23 @@ def test_residue_chunked_depth(mdg, orc, torch_cuda):
 
 
 
-@pytest.mark.parametrize("L", [2, 4])
-def test_residue_engines_agree_bitwise(mdg, orc, torch_cuda, L):
-    # the residue products are exact integers on every engine
-    m, n, k = 200, 150, 300
-    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 27, -20, 20))
-    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 28, -20, 20))
+@pytest.mark.parametrize("L,m,n,k", [(2, 200, 150, 300), (4, 200, 150, 300), (2, 600, 530, 1100),
+                                     (4, 257, 3, 129), (8, 1, 300, 5)])
+def test_residue_engines_agree_bitwise(mdg, orc, torch_cuda, L, m, n, k):
+    # the residue products are exact integers on every engine: the default
+    # hand-written kernel (backend 2; 256 x 256 tiles, ragged edges, several
+    # tiles, one-row and one-column tiles) against rocBLAS and hipBLASLt
+    print(f"residue engines L={L} {m}x{n}x{k}", flush=True)
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 27 + m, -20, 20))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 28 + n, -20, 20))
     out = {}
-    for engine in (1, 3, 0):
+    for engine, want in ((1, 1), (3, 3), (0, 2)):
         mdg.set_int8_backend(engine)
         try:
             C, info = mdg.gemm(L, m, n, k, A, B, method="int8_residue")
         finally:
             mdg.set_int8_backend(0)
-        assert info["backend"] == (engine or info["backend"]) and info["backend"] in (1, 3)
+        assert info["backend"] == want, (engine, info)
         out[engine] = to_host(C).view(np.uint64)
     assert np.array_equal(out[1], out[3]) and np.array_equal(out[1], out[0])
 
