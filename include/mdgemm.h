@@ -76,8 +76,9 @@ typedef struct mdg_gemm_info {
        profiling is on (mdg_set_profiling), else 0:
        [0] exponent scan, [1] slicing, [2] diagonal GEMMs, [3] reconstruction */
     float stage_ms[4];
-    int backend;           /* GEMM engine: 1 rocBLAS, 3 hipBLASLt (INT8 only),
-                              2 this library's MFMA kernels */
+    int backend;           /* GEMM engine that ran: 2 this library's MFMA
+                              kernels, 4 the 256 x 256 single-diagonal INT8
+                              kernel, 1 rocBLAS, 3 hipBLASLt (INT8 only) */
 } mdg_gemm_info;
 
 /* One problem of a batch (all pointers on the device). */
@@ -107,11 +108,13 @@ int mdg_release_workspace(void);
 int mdg_set_profiling(int on);
 
 /* Engine of the INT8 GEMMs (slice planes and residue products):
-   0 = autotuned per shape (the first call of a shape times the engines and
-   keeps the fastest), 1 = rocBLAS, 2 = this library's MFMA kernel (slice
-   planes; the residue products then autotune between 1 and 3),
-   3 = hipBLASLt (its fastest proposed algorithm).  All are exact and give
-   identical results; they differ only in speed.  Process-wide. */
+   0 or 2 = this library's MFMA kernels (the default: the band kernel for
+   slice planes, the plain INT8 GEMM for residue products), 1 = rocBLAS,
+   3 = hipBLASLt (the first call of a shape times the algorithms it proposes
+   and keeps the fastest), 4 = the 256 x 256 single-diagonal kernel for
+   slice planes (residue products as 0).  Engines 1, 3 and 4 are there for
+   comparison.  All are exact and give identical results; they differ only
+   in speed.  Process-wide. */
 int mdg_set_int8_backend(int backend);
 
 /* ----------------------------------------------------------------------

@@ -130,8 +130,8 @@ def set_profiling(on):
 
 
 def set_int8_backend(backend):
-    """INT8 GEMM engine: 0 autotuned, 1 rocBLAS, 2 hand-written MFMA kernel
-    (slice planes), 3 hipBLASLt."""
+    """INT8 GEMM engine: 0 or 2 this library's MFMA kernels (default), 1
+    rocBLAS, 3 hipBLASLt, 4 the 256 x 256 single-diagonal slice kernel."""
     _check(lib().mdg_set_int8_backend(int(backend)))
 
 
