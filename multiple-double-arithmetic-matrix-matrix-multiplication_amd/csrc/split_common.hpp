@@ -18,11 +18,12 @@ struct BlasHandle {
 };
 BlasHandle *blas_handle();
 
-// INT8 x INT8 -> INT32 strided-batched GEMM on rocBLAS or hipBLASLt, the
-// engine (and hipBLASLt algorithm) autotuned per shape unless forced by
-// mdg_set_int8_backend.  Column-major view: D_b (n x m, ldd) = Bt_b^T * A_b
-// with Bt_b[c * ldb + kk], A_b[r * lda + kk], kk < depth (row-major: the
-// m x n product of A_b and Bt_b^T).  *engine_used: 1 rocBLAS, 3 hipBLASLt.
+// INT8 x INT8 -> INT32 strided-batched GEMM on this library's MFMA kernel
+// (plain_i8_kernel), or on rocBLAS / hipBLASLt when mdg_set_int8_backend
+// picks them (or the shape does not fit the kernel).  Column-major view:
+// D_b (n x m, ldd) = Bt_b^T * A_b with Bt_b[c * ldb + kk], A_b[r * lda + kk],
+// kk < depth (row-major: the m x n product of A_b and Bt_b^T).
+// *engine_used: 2 the MFMA kernel, 1 rocBLAS, 3 hipBLASLt.
 int i8_gemm_tn(int batch, int m, int n, int depth, const int8_t *Bt, long long ldb, long long sb, const int8_t *A,
                long long lda, long long sa, int *D, long long ldd, long long sd, hipStream_t s, int *engine_used);
 
