@@ -235,17 +235,19 @@ __global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ 
 __device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
 
 // Entry (prob, tile row, tile column, item): one band item (i8band.hpp) of
-// one 128 x 128 tile of one problem; the item packed as g0 | t0 << 8 |
-// t1 << 16, the band width is the launch's G.  An item runs nkb k-blocks
-// times its slice steps, so every workgroup is deep enough to keep its copy
+// one 128 x 128 tile of one problem; the item packed as g0 & 0xff | t0 << 8
+// | t1 << 16 (g0 >= -(G-1) as a signed byte), the band width is the
+// launch's G; prob < 0: an empty entry.  An item runs nkb k-blocks times
+// its slice steps, so every workgroup is deep enough to keep its copy
 // pipeline full, however shallow the problem.
 template <int G>
 __global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, 2>()];
     const BEnt e = T[blockIdx.x];
+    if (e.prob < 0) return;
     const BProb &p = P[e.prob];
-    const int g0 = e.z & 0xff, t0 = (e.z >> 8) & 0xff, t1 = (e.z >> 16) & 0xff;
+    const int g0 = (int)(signed char)(e.z & 0xff), t0 = (e.z >> 8) & 0xff, t1 = (e.z >> 16) & 0xff;
     const int nkb = p.kp >> 7, Tc = p.kmax / p.kp;
     const long long tstride = (long long)nkb * p.S * BND_BLK;
     band_run<G, 2, 1>(lds, p.Aall + e.x * tstride, p.Ball + e.y * tstride, p.S, nkb, Tc, g0, t0, t1,
@@ -527,39 +529,58 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 const long long nb = ((long long)b.np * b.kp + 255) / 256;
                 for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
             });
-        // GEMM: one band-kernel launch per band width; items of all problems
-        // heaviest first (work = slice-pair blocks x tiles), their tiles
-        // next to each other
-        for (int G = 4; G >= 1; --G) {
-            struct Item {
-                int prob, code;
+        // GEMM: ONE band-kernel launch over the band items of every problem
+        // (band_items: all 4 diagonals wide).  Workgroups go to the 8 XCDs
+        // round-robin by their index, so entry 8 j + x runs on XCD x: the
+        // problems, heaviest first, are dealt to the XCDs in snake order
+        // (0..7, 7..0, ...), and each XCD's list holds its problems in that
+        // order, a problem's items heaviest first with each item's tiles next
+        // to each other -- the workgroups that read one problem's slices run
+        // on one XCD, close in time, and share its L2.  The lists advance at
+        // similar rates (problems of similar rank on each XCD); the shorter
+        // ones end with empty entries (prob = -1).
+        {
+            constexpr int NXCD = 8;
+            struct PW {
+                int prob;
                 long long work;
             };
-            std::vector<Item> items;
+            std::vector<PW> pws;
+            std::vector<std::vector<BandWork>> its(count);
             for (int i = 0; i < count; ++i) {
                 const BProb &b = pl.probs[i];
                 if (b.S <= 0 || b.m == 0 || b.n == 0 || b.k == 0) continue;
-                std::vector<BandWork> full, part;
-                int gpart = 0;
-                band_items(b.S, 4, b.kmax / b.kp, full, part, &gpart);
-                const std::vector<BandWork> &v = (G == 4) ? full : part;
-                if (G != 4 && gpart != G) continue;
-                const long long tiles = (long long)(b.mp / 128) * (b.np / 128);
-                for (const BandWork &w : v)
-                    items.push_back(Item{i, w.it.g0 | (w.it.t0 << 8) | (w.it.t1 << 16),
-                                         w.work * tiles * (b.kp / 128)});
+                band_items(b.S, 4, b.kmax / b.kp, its[i]);
+                std::stable_sort(its[i].begin(), its[i].end(),
+                                 [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
+                const long long per_item = (long long)(b.mp / 128) * (b.np / 128) * (b.kp / 128);
+                long long w = 0;
+                for (const BandWork &bw : its[i]) w += bw.work * per_item;
+                pws.push_back(PW{i, w});
             }
-            if (items.empty()) continue;
-            std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.work > y.work; });
-            Stage st{6, 0, (long long)pl.ents.size(), 0};
-            st.g = G;
-            for (const Item &it : items) {
-                const BProb &b = pl.probs[it.prob];
-                for (int x = 0; x < b.mp / 128; ++x)
-                    for (int y = 0; y < b.np / 128; ++y) push(it.prob, x, y, it.code);
+            std::stable_sort(pws.begin(), pws.end(), [](const PW &x, const PW &y) { return x.work > y.work; });
+            std::vector<std::vector<BEnt>> lists(NXCD);
+            for (size_t r = 0; r < pws.size(); ++r) {
+                const int round = (int)(r / NXCD), pos = (int)(r % NXCD);
+                const int x = (round & 1) ? NXCD - 1 - pos : pos;
+                const int i = pws[r].prob;
+                const BProb &b = pl.probs[i];
+                for (const BandWork &bw : its[i]) {
+                    const int code = (bw.it.g0 & 0xff) | (bw.it.t0 << 8) | (bw.it.t1 << 16);
+                    for (int tx = 0; tx < b.mp / 128; ++tx)
+                        for (int ty = 0; ty < b.np / 128; ++ty) lists[x].push_back(BEnt{i, tx, ty, code});
+                }
             }
-            st.count = (long long)pl.ents.size() - st.first;
-            pl.stages.push_back(st);
+            size_t len = 0;
+            for (const auto &l : lists) len = std::max(len, l.size());
+            if (len) {
+                Stage st{6, 0, (long long)pl.ents.size(), 0};
+                st.g = 4;
+                for (size_t j = 0; j < len; ++j)
+                    for (int x = 0; x < NXCD; ++x) pl.ents.push_back(j < lists[x].size() ? lists[x][j] : BEnt{-1, 0, 0, 0});
+                st.count = (long long)pl.ents.size() - st.first;
+                pl.stages.push_back(st);
+            }
         }
         for (int L : Ls) {
             const size_t before = pl.stages.size();
@@ -635,14 +656,7 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
         default: hipLaunchKernelGGL((b_slice_kernel<16, true>), grid, blk, 0, s, P, T); break;
         }
         break;
-    case 6:
-        switch (st.g) {
-        case 4: hipLaunchKernelGGL(b_band_i8_kernel<4>, grid, blk, 0, s, P, T); break;
-        case 3: hipLaunchKernelGGL(b_band_i8_kernel<3>, grid, blk, 0, s, P, T); break;
-        case 2: hipLaunchKernelGGL(b_band_i8_kernel<2>, grid, blk, 0, s, P, T); break;
-        default: hipLaunchKernelGGL(b_band_i8_kernel<1>, grid, blk, 0, s, P, T); break;
-        }
-        break;
+    case 6: hipLaunchKernelGGL(b_band_i8_kernel<4>, grid, blk, 0, s, P, T); break;
     case 7: {
         auto rec = [&](auto lc) -> int {
             constexpr int L = decltype(lc)::value;

@@ -53,7 +53,9 @@ __host__ __device__ __forceinline__ long long bnd_block(int tile, int kb, int t,
 }
 
 // One work item of the band kernel: diagonals g0 .. g0+gb-1, slices t of A
-// in [t0, t1), writing chunk c = t0 / Tc of each diagonal's planes.
+// in [t0, t1), writing chunk c = t0 / Tc of each diagonal's planes.  g0 < 0
+// for the band of leftover diagonals (band_items): diagonals below 0 are
+// empty.
 struct BandItem {
     short g0, gb, t0, t1;
 };
@@ -521,29 +523,31 @@ __global__ __launch_bounds__(256) void band_bf16_kernel(const int8_t *__restrict
 
 // Band items: bands of G diagonals, each band's t-range cut into chunks of
 // Tc slices (work = slice-pair blocks, padding steps included).  When S is
-// not a multiple of G the narrower band (*gpart wide, in `part`) holds the
-// cheapest diagonals, 0 .. S%G - 1 (one slice pair for diagonal 0): a
-// narrow band reuses its operands less, so it should carry little work.
-// Bands of exactly G diagonals go to `full`.
+// not a multiple of G, the S % G leftover diagonals are the cheapest ones,
+// 0 .. S%G - 1 (one slice pair for diagonal 0: a band that reuses its
+// operands less should carry little work), and they run as a G-wide band
+// shifted down, g0 = S%G - G < 0: its first G - S%G diagonals are empty.
+// band_run handles that band unchanged -- slice u = g0 + j - t < 0 reads the
+// zero block, the INT8 step types leave those MFMAs out, the epilogue skips
+// diagonals below t0 -- so every item is G wide and one launch of the G
+// kernel covers all of them.
 struct BandWork {
     BandItem it;
     long long work;
 };
 
-inline void band_items(int S, int G, int Tc, std::vector<BandWork> &full, std::vector<BandWork> &part, int *gpart)
+inline void band_items(int S, int G, int Tc, std::vector<BandWork> &items)
 {
-    full.clear();
-    part.clear();
-    *gpart = S % G;
-    std::vector<std::pair<int, int>> bands;   // (first diagonal, width)
-    if (*gpart) bands.push_back({0, *gpart});
-    for (int g0 = *gpart; g0 < S; g0 += G) bands.push_back({g0, G});
-    for (const auto &bd : bands) {
-        const int g0 = bd.first, gb = bd.second;
-        for (int t0 = 0; t0 < g0 + gb; t0 += Tc) {
-            const int t1 = std::min(t0 + Tc, g0 + gb);
-            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + gb - 1 : g0 + gb - t0;
-            (gb == G ? full : part).push_back(BandWork{BandItem{(short)g0, (short)gb, (short)t0, (short)t1}, steps * gb});
+    items.clear();
+    const int gpart = S % G;
+    std::vector<int> starts;   // first diagonal of each band
+    if (gpart) starts.push_back(gpart - G);
+    for (int g0 = gpart; g0 < S; g0 += G) starts.push_back(g0);
+    for (const int g0 : starts) {
+        for (int t0 = 0; t0 < g0 + G; t0 += Tc) {
+            const int t1 = std::min(t0 + Tc, g0 + G);
+            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + G - 1 : g0 + G - t0;
+            items.push_back(BandWork{BandItem{(short)g0, (short)G, (short)t0, (short)t1}, steps * G});
         }
     }
 }

@@ -785,50 +785,40 @@ BlasHandle *blas_handle()
 // the vendor libraries, kept only as comparison engines.
 enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
 
-// Band tables of one problem (band_items), heaviest items first.
-static bool build_band_tables(int S, int G, int Tc, BandTable &full, BandTable &part, int *gpart)
+// Band table of one problem (band_items: every item G wide, the leftover
+// diagonals as a shifted band), heaviest items first.
+static bool build_band_table(int S, int G, int Tc, BandTable &tab)
 {
-    std::vector<BandWork> v[2];
-    band_items(S, G, Tc, v[0], v[1], gpart);
-    BandTable *tabs[2] = {&full, &part};
-    for (int c = 0; c < 2; ++c) {
-        if ((int)v[c].size() > BND_MAX_ITEMS) return false;
-        std::stable_sort(v[c].begin(), v[c].end(), [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
-        tabs[c]->count = (int)v[c].size();
-        for (int i = 0; i < tabs[c]->count; ++i) tabs[c]->item[i] = v[c][i].it;
-    }
+    std::vector<BandWork> v;
+    band_items(S, G, Tc, v);
+    if ((int)v.size() > BND_MAX_ITEMS) return false;
+    std::stable_sort(v.begin(), v.end(), [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
+    tab.count = (int)v.size();
+    for (int i = 0; i < tab.count; ++i) tab.item[i] = v[i].it;
     return true;
 }
 
-// launch the band kernel over one table (G = gb of its items)
-static int launch_band(int gb, const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
+// one launch of the band kernel over the problem's table: bands of 4
+// diagonals on INT8 slices
+static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
                        int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    switch (gb) {
-    case 4: hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 3: hipLaunchKernelGGL((band_i8_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 2: hipLaunchKernelGGL((band_i8_kernel<2, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 1: hipLaunchKernelGGL((band_i8_kernel<1, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    default: set_error("band width %d", gb); return MDG_ERROR_INVALID_VALUE;
-    }
+    hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m,
+                       n, tab, D);
     MDG_LAUNCH_CHECK("band_i8_kernel");
     return MDG_SUCCESS;
 }
 
-// the same over blocked BF16 slices (nkb = kp / 64)
-static int launch_band_bf16(int gb, const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb,
-                            int Tc, int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
+// the same over blocked BF16 slices (nkb = kp / 64), bands of 3 diagonals
+static int launch_band_bf16(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
+                            int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    switch (gb) {
-    case 3: hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 2: hipLaunchKernelGGL((band_bf16_kernel<2, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    case 1: hipLaunchKernelGGL((band_bf16_kernel<1, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m, n, tab, D); break;
-    default: set_error("band width %d", gb); return MDG_ERROR_INVALID_VALUE;
-    }
+    hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n,
+                       m, n, tab, D);
     MDG_LAUNCH_CHECK("band_bf16_kernel");
     return MDG_SUCCESS;
 }
@@ -1037,11 +1027,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         // inside a no-allocation call only the hand-written kernels run
         const int forced = noalloc_active() ? 0 : int8_backend();
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
-        constexpr int BAND_G = 4;
-        BandTable tab_full, tab_part;
-        int gpart = 0;
-        if (engine == I8E_BAND && (kp > 131071 || !build_band_tables(S, BAND_G, kmax / kp, tab_full, tab_part, &gpart)))
-            engine = I8E_TILE256;
+        BandTable tab;
+        if (engine == I8E_BAND && (kp > 131071 || !build_band_table(S, 4, kmax / kp, tab))) engine = I8E_TILE256;
         int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
         MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
         if (engine == I8E_BAND) {
@@ -1056,8 +1043,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                                dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-            rc = launch_band(BAND_G, tab_full, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
-            if (!rc) rc = launch_band(gpart, tab_part, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+            rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
             if (rc) return rc;
         } else {
             // row-major K-contiguous slices: A_all[r][t kp + kk], Bt_all[c][(S-1-u) kp + kk]
@@ -1104,10 +1090,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
         // bands of 3 diagonals: 192 FP32 sums and 192 INT32 totals per lane
-        constexpr int BAND_G = 3;
-        BandTable tab_full, tab_part;
-        int gpart = 0;
-        const bool fits = build_band_tables(S, BAND_G, kmax / kp, tab_full, tab_part, &gpart);
+        BandTable tab;
+        const bool fits = build_band_table(S, 3, kmax / kp, tab);
         MDG_REQUIRE(fits, MDG_ERROR_RANGE, "too many BF16 band items for %d slices", S);
         int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
         const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
@@ -1120,9 +1104,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                            dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
         tm.mark(2);
         const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-        rc = launch_band_bf16(BAND_G, tab_full, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
-        if (!rc)
-            rc = launch_band_bf16(gpart, tab_part, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+        rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
         if (rc) return rc;
         p.backend = I8E_BAND;
         tm.mark(3);

@@ -22,6 +22,15 @@ def test_compat_symbols_exported(mdg):
         assert name in out, name
 
 
+def test_reference_kernels_exported(mdg):
+    """The __global__ ddmm/qdmm/odmm/hdmm of include/mdgemm_kernels.hpp: their
+    host-side launch stubs are exported, so hipcc-built callers link."""
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", mdg.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for name in ("ddmm", "qdmm", "odmm", "hdmm"):
+        assert f"__device_stub__{name}(double*, double*, double*, int)" in out, name
+
+
 @pytest.mark.parametrize("L", [0, 1, 3, 5, 32])
 def test_bad_limb_count_rejected(mdg, L):
     lib = mdg.lib()

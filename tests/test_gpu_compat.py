@@ -57,3 +57,25 @@ def test_compat_driver(orc, tmp_path, p, n):
             assert_bits_equal(ld[f"sq_2_{pp2}"], want2, f"pllsqueeze_2_{pp2}")
             assert_bits_equal(ld[f"sqnt_2_{pp2}"], want2, f"pllntsqueeze_2_{pp2}")
         assert_bits_equal(ld["sq_12"], orc.squeeze(2, 12, ld["x12"]), "pllsqueeze(x, 2, 12)")
+
+
+REF_KERNELS_EXE = os.path.join(ROOT, "tests", "cpp", "ref_kernels_test")
+
+
+@pytest.mark.parametrize("p,n,nfrag", [(2, 64, 16), (4, 64, 8), (8, 48, 16), (16, 32, 32), (4, 96, 32)])
+def test_exported_kernels_reference_launch(orc, tmp_path, p, n, nfrag):
+    """The exported __global__ ddmm/qdmm/odmm/hdmm (C++/TCcuda.h:62-65),
+    launched by a HIP program with the reference's grid Gr(n/nfrag,
+    n/nfrag) x Bl(nfrag, nfrag) (C++/TCcuda.cu:265-293): bit-identical to
+    the oracle's multi-double GEMM with FMA TwoProd."""
+    assert os.path.exists(REF_KERNELS_EXE), "tests/cpp/ref_kernels_test is not built; run __graft_entry__.build()"
+    A = orc.generate(orc.GEN_QD, p, n, n, 700 + p, -8, 8)
+    B = orc.generate(orc.GEN_QD, p, n, n, 800 + p, -8, 8)
+    A.tofile(tmp_path / "A.bin")
+    B.tofile(tmp_path / "B.bin")
+    r = subprocess.run([REF_KERNELS_EXE, str(p), str(n), str(nfrag), str(tmp_path / "A.bin"),
+                        str(tmp_path / "B.bin"), str(tmp_path / "C.bin")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    C = np.fromfile(tmp_path / "C.bin", dtype=np.float64)
+    assert_bits_equal(C, orc.gemm(p, n, n, n, A, B, mode=orc.TP_FMA), f"exported kernel p={p}")
