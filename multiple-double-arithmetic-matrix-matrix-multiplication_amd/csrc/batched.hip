@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "blkslice.hpp"
 #include "i8band.hpp"
 #include "i8tile.hpp"
 #include "md_arith.hpp"
@@ -200,33 +201,18 @@ __global__ void b_guard_kernel(const BProb *__restrict__ P, const BEnt *__restri
 }
 
 // --------------------------------------------------------------- slices --
-// slices in the blocked layout of the band kernel (i8band.hpp): thread per
-// entry (r, kk) of the padded mp x kp domain (A) or np x kp (B, "row" r =
-// column r of B), kk fastest; slice t lands in block (r / 128, kk / 128, t)
+// slices in the blocked layout of the band kernel (blkslice.hpp): entry
+// (prob, row tile x, k-block y), blk_rows_per_tile(S) rows (A) or columns (B)
+// of 128 k-bytes each
 template <int L, bool TRANS>
 __global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
-    constexpr int NW = FixedWords<L>::NW;
+    extern __shared__ __attribute__((aligned(16))) int8_t stage[];   // the stage's largest blk_stage_bytes
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
-    const int rows = TRANS ? p.n : p.m, rp = TRANS ? p.np : p.mp;
-    const long long idx = (long long)e.x * 256 + threadIdx.x;
-    if (idx >= (long long)rp * p.kp) return;
-    const int r = (int)(idx / p.kp);
-    const int kk = (int)(idx - (long long)r * p.kp);
-    int8_t *o = (TRANS ? p.Ball : p.Aall) + bnd_block(r >> 7, kk >> 7, 0, p.S, p.kp >> 7) + bnd_pos(r & 127, kk & 127);
-    const int ex = TRANS ? p.F[r < rows ? r : 0] : p.E[r < rows ? r : 0];
-    if (r >= rows || kk >= p.k || ex == BZERO_SCALE) {
-        for (int t = 0; t < p.S; ++t) o[(long long)t * BND_BLK] = 0;
-        return;
-    }
-    double x[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-        x[l] = TRANS ? p.B[((long long)kk * p.ldb + r) * L + l] : p.A[((long long)r * p.lda + kk) * L + l];
-    uint32_t N[NW];
-    to_fixed<L, NW>(x, p.S * p.w - 2 - ex, N);
-    fixed_digits<NW>(N, p.S, p.w, [&](int t, int d) { o[(long long)t * BND_BLK] = (int8_t)d; });
+    slice_tile<L, TRANS, int8_t>(TRANS ? p.n : p.m, p.k, p.kp, TRANS ? p.B : p.A, TRANS ? p.ldb : p.lda,
+                                 TRANS ? p.F : p.E, BZERO_SCALE, p.S, p.w, e.y, e.x * blk_rows_per_tile(p.S),
+                                 TRANS ? p.Ball : p.Aall, stage);
 }
 
 // ------------------------------------------------------- INT8 band tiles --
@@ -356,6 +342,7 @@ struct Stage {
     bool chunked = false;   // reconstruction: some problem has K-chunk planes
     int tm = 128;  // GEMM: tile rows and columns
     int g = 0;     // GEMM: band width of the band-kernel launch
+    int lds = 0;   // slicing: dynamic LDS bytes (the largest any problem of the stage needs)
 };
 
 struct Plan {
@@ -519,16 +506,17 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         if (rc) return rc;
     }
     if (i8) {
-        for (int L : Ls)
-            add_stage(4, L, [&](int i, const BProb &b) {
-                const long long nb = ((long long)b.mp * b.kp + 255) / 256;
-                for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
-            });
-        for (int L : Ls)
-            add_stage(5, L, [&](int i, const BProb &b) {
-                const long long nb = ((long long)b.np * b.kp + 255) / 256;
-                for (long long x = 0; x < nb; ++x) push(i, (int)x, 0, 0);
-            });
+        for (int kind : {4, 5})
+            for (int L : Ls) {
+                int lds = 0;
+                const size_t before = pl.stages.size();
+                add_stage(kind, L, [&](int i, const BProb &b) {
+                    lds = std::max(lds, blk_stage_bytes(b.S));
+                    for (int x = 0; x < (kind == 4 ? b.mp : b.np) / blk_rows_per_tile(b.S); ++x)
+                        for (int y = 0; y < b.kp / 128; ++y) push(i, x, y, 0);
+                });
+                if (pl.stages.size() > before) pl.stages.back().lds = lds;
+            }
         // GEMM: ONE band-kernel launch over the band items of every problem
         // (band_items: all 4 diagonals wide).  Workgroups go to the 8 XCDs
         // round-robin by their index, so entry 8 j + x runs on XCD x: the
@@ -642,18 +630,18 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 9: hipLaunchKernelGGL(b_guard_kernel, grid, blk, 0, s, P, T); break;
     case 4:
         switch (st.L) {
-        case 2: hipLaunchKernelGGL((b_slice_kernel<2, false>), grid, blk, 0, s, P, T); break;
-        case 4: hipLaunchKernelGGL((b_slice_kernel<4, false>), grid, blk, 0, s, P, T); break;
-        case 8: hipLaunchKernelGGL((b_slice_kernel<8, false>), grid, blk, 0, s, P, T); break;
-        default: hipLaunchKernelGGL((b_slice_kernel<16, false>), grid, blk, 0, s, P, T); break;
+        case 2: hipLaunchKernelGGL((b_slice_kernel<2, false>), grid, blk, st.lds, s, P, T); break;
+        case 4: hipLaunchKernelGGL((b_slice_kernel<4, false>), grid, blk, st.lds, s, P, T); break;
+        case 8: hipLaunchKernelGGL((b_slice_kernel<8, false>), grid, blk, st.lds, s, P, T); break;
+        default: hipLaunchKernelGGL((b_slice_kernel<16, false>), grid, blk, st.lds, s, P, T); break;
         }
         break;
     case 5:
         switch (st.L) {
-        case 2: hipLaunchKernelGGL((b_slice_kernel<2, true>), grid, blk, 0, s, P, T); break;
-        case 4: hipLaunchKernelGGL((b_slice_kernel<4, true>), grid, blk, 0, s, P, T); break;
-        case 8: hipLaunchKernelGGL((b_slice_kernel<8, true>), grid, blk, 0, s, P, T); break;
-        default: hipLaunchKernelGGL((b_slice_kernel<16, true>), grid, blk, 0, s, P, T); break;
+        case 2: hipLaunchKernelGGL((b_slice_kernel<2, true>), grid, blk, st.lds, s, P, T); break;
+        case 4: hipLaunchKernelGGL((b_slice_kernel<4, true>), grid, blk, st.lds, s, P, T); break;
+        case 8: hipLaunchKernelGGL((b_slice_kernel<8, true>), grid, blk, st.lds, s, P, T); break;
+        default: hipLaunchKernelGGL((b_slice_kernel<16, true>), grid, blk, st.lds, s, P, T); break;
         }
         break;
     case 6: hipLaunchKernelGGL(b_band_i8_kernel<4>, grid, blk, 0, s, P, T); break;

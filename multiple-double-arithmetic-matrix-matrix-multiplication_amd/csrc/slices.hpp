@@ -28,36 +28,43 @@ struct FixedWords {
     static constexpr int NW = (L == 2) ? 8 : (L == 4) ? 12 : (L == 8) ? 20 : 36;
 };
 
-// decompose a finite double: x = m * 2^e, |m| < 2^53 (m = 0 for zero)
+// decompose a finite double: x = m * 2^e, |m| < 2^53 (m = 0 for zero), read
+// from the bit fields (normal: the 53-bit significand with its implicit bit
+// and e = biased exponent - 1075; subnormal: the raw 52 bits and e = -1074)
 __device__ __forceinline__ void limb_decomp(double x, long long &m, int &e)
 {
-    int ex = 0;
-    double fr = frexp(x, &ex);            // |fr| in [0.5, 1) or 0
-    m = (long long)ldexp(fr, 53);
-    e = ex - 53;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const int ex = (int)((u >> 52) & 0x7ff);
+    const long long mant = (long long)((u & ((1ull << 52) - 1)) | ((ex != 0) ? (1ull << 52) : 0ull));
+    e = ((ex != 0) ? ex : 1) - 1075;
+    m = (u >> 63) ? -mant : mant;
 }
 
-// N += m * 2^p (p may be negative: floor), sign-extended over NW words
+// N += floor(m * 2^p), two's complement over NW words.  m 2^p = M 2^(32 j0)
+// with b = p mod 32, j0 = floor(p / 32) and M = m 2^b: M's words w0, w1
+// (bits 0..63) and w2 (bits 64..95, sign-extended), then its sign s in
+// every word above.  Word j of N adds the word j - j0 of M (words below 0,
+// the bits below N's bit 0, drop out: floor); straight-line selects and one
+// add-with-carry chain, no branches.
 template <int NW>
 __device__ __forceinline__ void fixed_add(uint32_t (&N)[NW], long long m, int p)
 {
-    if (m == 0) return;
-    uint64_t carry = 0;
+    const int b = p & 31, j0 = p >> 5;
+    const unsigned long long lo = (unsigned long long)m << b;
+    const uint32_t s = (uint32_t)(m >> 63);
+    const uint32_t w0 = (uint32_t)lo, w1 = (uint32_t)(lo >> 32);
+    const uint32_t w2 = (uint32_t)((m >> 32) >> (32 - b));   // b = 0: the sign word
+    unsigned int carry = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-        const int q = 32 * j - p;      // bit of m landing on bit 0 of word j
-        uint32_t chunk;
-        if (q >= 63)
-            chunk = (uint32_t)(m >> 63);
-        else if (q >= 0)
-            chunk = (uint32_t)(m >> q);
-        else if (q > -32)
-            chunk = (uint32_t)((uint64_t)m << (-q));
-        else
-            chunk = 0u;
-        const uint64_t t = (uint64_t)N[j] + chunk + carry;
-        N[j] = (uint32_t)t;
-        carry = t >> 32;
+        const int d = j - j0;
+        uint32_t a = (d >= 3) ? s : 0u;
+        a = (d == 2) ? w2 : a;
+        a = (d == 1) ? w1 : a;
+        a = (d == 0) ? w0 : a;
+        unsigned int co;
+        N[j] = __builtin_addc(N[j], a, carry, &co);
+        carry = co;
     }
 }
 
@@ -133,6 +140,40 @@ __device__ __forceinline__ void fixed_digits(const uint32_t (&N)[NW], int S, int
     int top = (int)raw;
     if (raw & (uint32_t)half) top -= full;
     emit(0, top + carry);
+}
+
+// w = 8 digits, the same as fixed_digits(N, S, 8, emit): with N' = N +
+// sum_{i < S-1} 128 * 256^i, byte i of N' is d_i + 128 for the balanced
+// digits d_i (i = 0 least significant) -- as an INT8, d_i = byte i xor 0x80
+// -- and the top digit is N' >> 8 (S-1), a signed byte here (|N| < 2^(8S-2)).
+// emit(t, d) receives slice t = S-1-i.  S must be uniform across the wave
+// (the byte loop bounds are branches on it).
+template <int NW, typename Emit>
+__device__ __forceinline__ void fixed_digits8(uint32_t (&N)[NW], int S, Emit emit)
+{
+    const int nb = S - 1;   // bytes that carry the +128 offset
+    unsigned int carry = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+        const int k = nb - 4 * j;   // offset bytes in word j
+        const uint32_t add = (k >= 4) ? 0x80808080u : ((k > 0) ? (0x80808080u >> (8 * (4 - k))) : 0u);
+        unsigned int co;
+        N[j] = __builtin_addc(N[j], add, carry, &co);
+        carry = co;
+    }
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+        if (4 * j > nb) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * j + q;
+            const int byte = (int)((N[j] >> (8 * q)) & 0xffu);
+            if (i < nb)
+                emit(nb - i, byte - 128);
+            else if (i == nb)
+                emit(0, (byte ^ 0x80) - 128);   // the signed byte
+        }
+    }
 }
 
 }  // namespace mdg

@@ -44,6 +44,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "blkslice.hpp"
 #include "i8band.hpp"
 #include "md_arith.hpp"
 #include "recon.hpp"
@@ -246,35 +247,25 @@ __global__ __launch_bounds__(256) void slice_b_kernel(int k, int n, int kp, int 
 }
 
 // INT8 (or BF16) slices in the blocked layout of the band kernel
-// (i8band.hpp): thread per entry (r, kk) of the padded rp x kp domain, kk
-// fastest; a block row holds EB = 128 / sizeof(T) entries, so slice t of the
-// entry lands in block (r / 128, kk / EB, t) at byte bnd_pos(r % 128,
-// sizeof(T) (kk % EB)).  TRANS: the operand is B (k x n, row stride ldx) and
-// "row" r is column r of B.
+// (i8band.hpp), one tile of blk_rows_per_tile(S) rows x one k-block per
+// workgroup (blkslice.hpp).  Grid: x = row tiles (rp / RT), y = k-blocks
+// (kp / EB, EB = 128 / sizeof(T)).  TRANS: the operand is B (k x n, row
+// stride ldx) and "row" r is column r of B.
 template <int L, bool TRANS, typename T = int8_t>
-__global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int rp, int kp, const double *__restrict__ X,
+__global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int kp, const double *__restrict__ X,
                                                         long long ldx, const int *__restrict__ E, int S, int w,
                                                         int8_t *__restrict__ out)
 {
-    constexpr int NW = FixedWords<L>::NW;
-    constexpr int EB = BND_BK / (int)sizeof(T);
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)rp * kp) return;
-    const int r = (int)(idx / kp);
-    const int kk = (int)(idx - (long long)r * kp);
-    T *o = (T *)(out + bnd_block(r >> 7, kk / EB, 0, S, kp / EB) + bnd_pos(r & 127, (int)sizeof(T) * (kk % EB)));
-    constexpr long long TS = BND_BLK / (int)sizeof(T);   // slice stride in T
-    if (r >= rows || kk >= k || E[r] == ZERO_SCALE) {
-        for (int t = 0; t < S; ++t) o[(long long)t * TS] = digit_as<T>(0);
-        return;
-    }
-    double x[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-        x[l] = TRANS ? X[((long long)kk * ldx + r) * L + l] : X[((long long)r * ldx + kk) * L + l];
-    uint32_t N[NW];
-    to_fixed<L, NW>(x, S * w - 2 - E[r], N);
-    fixed_digits<NW>(N, S, w, [&](int t, int d) { o[(long long)t * TS] = digit_as<T>(d); });
+    extern __shared__ __attribute__((aligned(16))) int8_t stage[];   // blk_stage_bytes(S, sizeof(T))
+    slice_tile<L, TRANS, T>(rows, k, kp, X, ldx, E, ZERO_SCALE, S, w, blockIdx.y,
+                            blockIdx.x * blk_rows_per_tile(S, (int)sizeof(T)), out, stage);
+}
+
+// grid of slice_blk_kernel for rp padded rows
+template <typename T>
+static dim3 slice_blk_grid(int rp, int kp, int S)
+{
+    return dim3((unsigned)(rp / blk_rows_per_tile(S, (int)sizeof(T))), (unsigned)(kp / (BND_BK / (int)sizeof(T))));
 }
 
 // --------------------------------------------------- INT8 diagonal GEMM --
@@ -1037,10 +1028,12 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp, 2);
             int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp, 3);
             MDG_REQUIRE(Aall && Ball, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-            hipLaunchKernelGGL((slice_blk_kernel<L, false>), dim3(blocks_for<int>((long long)mpb * kp, 256)),
-                               dim3(256), 0, s, m, k, mpb, kp, A, lda, E, S, w, Aall);
-            hipLaunchKernelGGL((slice_blk_kernel<L, true>), dim3(blocks_for<int>((long long)npb * kp, 256)),
-                               dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
+            hipLaunchKernelGGL((slice_blk_kernel<L, false>), slice_blk_grid<int8_t>(mpb, kp, S), dim3(256),
+                               blk_stage_bytes(S), s, m, k,
+                               kp, A, lda, E, S, w, Aall);
+            hipLaunchKernelGGL((slice_blk_kernel<L, true>), slice_blk_grid<int8_t>(npb, kp, S), dim3(256),
+                               blk_stage_bytes(S), s, n, k,
+                               kp, B, ldb, F, S, w, Ball);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
             rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
@@ -1098,10 +1091,12 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
         int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp * 2, 3);
         MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-        hipLaunchKernelGGL((slice_blk_kernel<L, false, __bf16>), dim3(blocks_for<int>((long long)mpb * kp, 256)),
-                           dim3(256), 0, s, m, k, mpb, kp, A, lda, E, S, w, Aall);
-        hipLaunchKernelGGL((slice_blk_kernel<L, true, __bf16>), dim3(blocks_for<int>((long long)npb * kp, 256)),
-                           dim3(256), 0, s, n, k, npb, kp, B, ldb, F, S, w, Ball);
+        hipLaunchKernelGGL((slice_blk_kernel<L, false, __bf16>), slice_blk_grid<__bf16>(mpb, kp, S), dim3(256),
+                           blk_stage_bytes(S, 2), s, m,
+                           k, kp, A, lda, E, S, w, Aall);
+        hipLaunchKernelGGL((slice_blk_kernel<L, true, __bf16>), slice_blk_grid<__bf16>(npb, kp, S), dim3(256),
+                           blk_stage_bytes(S, 2), s, n,
+                           k, kp, B, ldb, F, S, w, Ball);
         tm.mark(2);
         const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
         rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
