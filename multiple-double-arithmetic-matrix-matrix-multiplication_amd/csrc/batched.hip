@@ -236,8 +236,16 @@ __global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict_
     const int g0 = (int)(signed char)(e.z & 0xff), t0 = (e.z >> 8) & 0xff, t1 = (e.z >> 16) & 0xff;
     const int nkb = p.kp >> 7, Tc = p.kmax / p.kp;
     const long long tstride = (long long)nkb * p.S * BND_BLK;
-    band_run<G, 2, 1>(lds, p.Aall + e.x * tstride, p.Ball + e.y * tstride, p.S, nkb, Tc, g0, t0, t1,
-                      (long long)e.x * BND_TM, (long long)e.y * BND_TM, p.m, p.n, p.D);
+    const int8_t *At = p.Aall + e.x * tstride, *Bt = p.Ball + e.y * tstride;
+    const long long row0 = (long long)e.x * BND_TM, col0 = (long long)e.y * BND_TM;
+    // 32-row MFMA tiles that hold rows below m: the others only multiply the
+    // zero padding of the tile and are left out
+    switch ((int)min(4ll, (p.m - row0 + 31) >> 5)) {
+    case 1: band_run<G, 2, 1, false, 1>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
+    case 2: band_run<G, 2, 1, false, 2>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
+    case 3: band_run<G, 2, 1, false, 3>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
+    default: band_run<G, 2, 1, false, 4>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
+    }
 }
 
 // ------------------------------------------------------- reconstruction --

@@ -157,8 +157,11 @@ constexpr int bnd_lds_bytes()
 // One band item of one output tile: Atile / Btile = the tile's blocked
 // slices (block (kb, slice) at (kb * S + slice) * 16 KiB), diagonals
 // g0 .. g0+G-1, slices t of A in [t0, t1); the tile's rows start at row0 of
-// the m x n planes, its columns at col0.  lds: bnd_lds_bytes<G, PA>() bytes.
-template <int G, int PA, int WM, bool BF = false>
+// the m x n planes, its columns at col0.  lds: bnd_lds_bytes<G, PA>()
+// bytes.  IMV < IM (INT8) keeps only the first IMV 32-row MFMA tiles of
+// every wave: the work on the zero padding of a tile whose rows from
+// row0 + 32 IMV on are beyond m is left out.
+template <int G, int PA, int WM, bool BF = false, int IMV = BND_TM / WM / 32>
 __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__ Atile,
                                          const int8_t *__restrict__ Btile, int S, int nkb, int Tc, int g0, int t0,
                                          int t1, long long row0, long long col0, int m, int n, int *__restrict__ D)
@@ -167,6 +170,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     // WM x WN waves, each TMW x TNW of every diagonal: IM x IN MFMA tiles
     constexpr int WN = 4 / WM, TMW = BND_TM / WM, TNW = BND_TM / WN, IM = TMW / 32, IN = TNW / 32;
     static_assert(bnd_lds_bytes<G, PA>() <= 163840, "LDS ring exceeds 160 KiB");
+    static_assert(IMV >= 1 && IMV <= IM && (!BF || IMV == IM), "row tiles: all of them for BF16");
     constexpr int gb = G;   // items of one launch have exactly G diagonals
     const int nA = t1 - t0;
     const int nsk = (t1 <= g0 + 1) ? nA + gb - 1 : g0 + gb - t0;
@@ -298,7 +302,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     auto fetch = [&](int buf, int ks, auto jc) {
         constexpr int J0 = decltype(jc)::value;
 #pragma unroll
-        for (int i = 0; i < IM; ++i) a[buf][i] = *(const bnd_i4 *)(pa + i * 4096 + koff[ks]);
+        for (int i = 0; i < IMV; ++i) a[buf][i] = *(const bnd_i4 *)(pa + i * 4096 + koff[ks]);
 #pragma unroll
         for (int j = J0; j < G; ++j)
 #pragma unroll
@@ -315,7 +319,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         for (int idx = 0; idx < NT; ++idx) {
             const int T = (O + idx) % NT;
             const int j = T / (IM * IN), i = (T / IN) % IM, jj = T % IN;
-            if (j < J0) continue;
+            if (j < J0 || i >= IMV) continue;
             if constexpr (BF)
                 acc[j][i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                     __builtin_bit_cast(bnd_bf8, a[buf][i]), __builtin_bit_cast(bnd_bf8, b[buf][j][jj]),
@@ -353,7 +357,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         // the scheduler places one fragment read (or one global->LDS copy)
         // after each MFMA, so no burst of reads or copies leaves the MFMA
         // pipe idle: NR = fragment reads, NM = MFMAs per k-step
-        constexpr int NR = IM + IN * (G - J0), NM = IM * IN * (G - J0), NR0 = IM + IN * G;
+        constexpr int NR = IMV + IN * (G - J0), NM = IMV * IN * (G - J0), NR0 = IMV + IN * G;
 #pragma unroll
         for (int ks = 0; ks < 3; ++ks) {
             fetch((ks & 1) ^ 1, ks + 1, jc);
@@ -447,6 +451,9 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         const int nfull = max(0, min(nA, g0 + 1 - t0));
         const int jlo = max(1, t0 + nfull - g0), jhi = min(G - 1, t1 - 1 - g0);
         const int npad = nsk - nA;
+        // (a second instantiation of these steps for a last, partial
+        // k-block with fewer k-steps made the compiler spill the
+        // accumulators: every k-block runs all four k-steps)
         for (int kb = 0; kb < nkb; ++kb) {
             for (int c = 0; c < nfull; ++c) step(NoZero{}, NoZero{}, J0All{});
             bnd_unroll(std::make_integer_sequence<int, G - 1>{}, [&](auto rc) {
@@ -469,7 +476,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         if (t0 > g) continue;   // this chunk adds nothing to diagonal g
         int *plane = D + (long long)(bnd_first_plane(g, Tc) + c) * mn;
 #pragma unroll
-        for (int i = 0; i < IM; ++i)
+        for (int i = 0; i < IMV; ++i)
 #pragma unroll
             for (int jj = 0; jj < IN; ++jj) {
                 const long long col = col0 + wn * TNW + jj * 32 + (lane & 31);
