@@ -265,23 +265,30 @@ __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restr
     const int er = p.E[r], fc = p.F[c];
     if (er != BZERO_SCALE && fc != BZERO_SCALE) {
         const long long mn = (long long)p.m * p.n;
-        diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
-                         [&](int g) {
-                             if constexpr (!CHUNKED) return (long long)p.D[(long long)g * mn + idx];
-                             int first;
-                             if (p.kmax % p.kp == 0) {
-                                 const int cc = p.kmax / p.kp, q = g / cc, rr = g - q * cc;
-                                 first = cc * q * (q + 1) / 2 + rr * (q + 1);
-                             } else {
-                                 first = 0;
-                                 for (int h = 0; h < g; ++h) first += b_chunks(h, p.kp, p.kmax);
-                             }
-                             const int nc = b_chunks(g, p.kp, p.kmax);
-                             long long v = 0;
-                             for (int q = 0; q < nc; ++q) v += p.D[(long long)(first + q) * mn + idx];
-                             return v;
-                         },
-                         y);
+        // the planes live in global memory (a generic pointer read from the
+        // problem table would make every load a flat one)
+        const __attribute__((address_space(1))) int *D = (const __attribute__((address_space(1))) int *)p.D;
+        if constexpr (!CHUNKED) {
+            diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
+                                [&](int g) { return (int)D[(long long)g * mn + idx]; }, y);
+        } else {
+            diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
+                                [&](int g) {
+                                    int first;
+                                    if (p.kmax % p.kp == 0) {
+                                        const int cc = p.kmax / p.kp, q = g / cc, rr = g - q * cc;
+                                        first = cc * q * (q + 1) / 2 + rr * (q + 1);
+                                    } else {
+                                        first = 0;
+                                        for (int h = 0; h < g; ++h) first += b_chunks(h, p.kp, p.kmax);
+                                    }
+                                    const int nc = b_chunks(g, p.kp, p.kmax);
+                                    long long v = 0;
+                                    for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
+                                    return v;
+                                },
+                                y);
+        }
     }
     double *o = p.C + (r * p.ldc + c) * L;
 #pragma unroll

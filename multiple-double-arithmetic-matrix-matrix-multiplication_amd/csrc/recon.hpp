@@ -59,12 +59,17 @@ __device__ __forceinline__ void top_pieces(const double (&d)[NP], int tp, double
     for (int s = 0; s < K; ++s) out[s] = (s < NP) ? r[s] : 0.0;
 }
 
-// D(g) returns the INT64 diagonal sum g (0 <= g < S; it is called with every
-// index of the word layout, clamped into range, so that all loads issue as
-// straight-line code); GS = diag_group(w); y receives the L limbs
+// D(g) returns the diagonal sum g (0 <= g < S; it is called with every index
+// of the word layout, clamped into range, so that all loads issue as
+// straight-line code) as an INT64, or as an INT32 when every sum fits one
+// (an unchunked INT8 plane); GS = diag_group(w); y receives the L limbs
 template <int L, int NW, int GS, typename DiagFn>
 __device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, double (&y)[L])
 {
+    // INT32 sums: D_g 2^sh with sh <= w (GS-1) stays below 2^47 and a word's
+    // GS of them below 2^49, so each sum is added whole to its own word
+    // (the carry pass below spreads it); INT64 sums are split at ws bits
+    constexpr bool NARROW = std::is_same_v<decltype(D(0)), int>;
     const int ws = w * GS;
     const int nt = (S + GS - 1) / GS, pad = nt * GS - S;
     long long z[NW];
@@ -78,13 +83,18 @@ __device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, doubl
             const int g = t * GS + u;
             // unconditional load (clamped index) and a mask, not a branch:
             // a branch around each load makes every load wait at its join
-            const long long on = -(long long)(t >= 0 && g < S);
-            const long long dg = D(min(max(g, 0), S - 1)) & on;
-            // D_g 2^sh, sh = w (GS-1-u): the bits below ws into word i, the
-            // rest (arithmetic shift) into word i+1 -- no overflow
             const int sh = w * (GS - 1 - u);
-            z[i] += (dg & ((1ll << (ws - sh)) - 1)) << sh;
-            if (i + 1 < NW) z[i + 1] += dg >> (ws - sh);
+            if constexpr (NARROW) {
+                const int dg = D(min(max(g, 0), S - 1)) & -(int)(t >= 0 && g < S);
+                z[i] += (long long)dg << sh;
+            } else {
+                const long long on = -(long long)(t >= 0 && g < S);
+                const long long dg = D(min(max(g, 0), S - 1)) & on;
+                // D_g 2^sh, sh = w (GS-1-u): the bits below ws into word i,
+                // the rest (arithmetic shift) into word i+1 -- no overflow
+                z[i] += (dg & ((1ll << (ws - sh)) - 1)) << sh;
+                if (i + 1 < NW) z[i + 1] += dg >> (ws - sh);
+            }
         }
         // the loads of RB words issue together; the barrier keeps the
         // scheduler from hoisting all of them (registers, then occupancy)
@@ -162,8 +172,8 @@ inline int with_words(int need, Fn fn)
     };
     using std::integral_constant;
     if constexpr (L == 2) {
-        if (pick(integral_constant<int, 16>()) || pick(integral_constant<int, 24>()) ||
-            pick(integral_constant<int, 32>()))
+        if (pick(integral_constant<int, 12>()) || pick(integral_constant<int, 16>()) ||
+            pick(integral_constant<int, 24>()) || pick(integral_constant<int, 32>()))
             return MDG_SUCCESS;
     } else if constexpr (L == 4) {
         if (pick(integral_constant<int, 16>()) || pick(integral_constant<int, 24>()) ||

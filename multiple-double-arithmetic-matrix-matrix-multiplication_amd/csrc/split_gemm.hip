@@ -726,19 +726,21 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
     const int er = E[r], fc = F[c];
     if (er != ZERO_SCALE && fc != ZERO_SCALE) {
         const long long mn = (long long)m * n;
-        diag_fold<L, NW, 3>(S, w, er + fc + 4 - w * (S + 1),
-                            [&](int g) {
-                                if constexpr (!CHUNKED) {
-                                    return (long long)D[(long long)g * mn + idx];
-                                } else {
+        if constexpr (!CHUNKED) {
+            // one INT32 plane per diagonal: the fold takes the sums as INT32
+            diag_fold<L, NW, 3>(S, w, er + fc + 4 - w * (S + 1), [&](int g) { return D[(long long)g * mn + idx]; },
+                                y);
+        } else {
+            diag_fold<L, NW, 3>(S, w, er + fc + 4 - w * (S + 1),
+                                [&](int g) {
                                     const int first = i8_first_plane(g, kp, kmax);
                                     const int nc = i8_chunks(g, kp, kmax);
                                     long long v = 0;
                                     for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
                                     return v;
-                                }
-                            },
-                            y);
+                                },
+                                y);
+        }
     }
     double *o = C + (r * ldc + c) * L;
 #pragma unroll
