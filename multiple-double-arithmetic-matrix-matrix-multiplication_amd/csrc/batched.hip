@@ -203,16 +203,21 @@ __global__ void b_guard_kernel(const BProb *__restrict__ P, const BEnt *__restri
 // --------------------------------------------------------------- slices --
 // slices in the blocked layout of the band kernel (blkslice.hpp): entry
 // (prob, row tile x, k-block y), blk_rows_per_tile(S) rows (A) or columns (B)
-// of 128 k-bytes each
+// of 128 k-bytes each; the padding of the plan's slices is zeroed when the
+// plan is built and never written again, so row tiles wholly past m (n) get
+// no entry and the digits past k are not stored
 template <int L, bool TRANS>
 __global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t stage[];   // the stage's largest blk_stage_bytes
     const BEnt e = T[blockIdx.x];
     const BProb &p = P[e.prob];
-    slice_tile<L, TRANS, int8_t>(TRANS ? p.n : p.m, p.k, p.kp, TRANS ? p.B : p.A, TRANS ? p.ldb : p.lda,
-                                 TRANS ? p.F : p.E, BZERO_SCALE, p.S, p.w, e.y, e.x * blk_rows_per_tile(p.S),
-                                 TRANS ? p.Ball : p.Aall, stage);
+    with_slice_words<L>(p.S, p.w, [&](auto nw) {
+        slice_tile<L, TRANS, int8_t, decltype(nw)::value, true>(TRANS ? p.n : p.m, p.k, p.kp, TRANS ? p.B : p.A,
+                                                          TRANS ? p.ldb : p.lda, TRANS ? p.F : p.E, BZERO_SCALE, p.S,
+                                                          p.w, e.y, e.x * blk_rows_per_tile(p.S),
+                                                          TRANS ? p.Ball : p.Aall, stage);
+    });
 }
 
 // ------------------------------------------------------- INT8 band tiles --
@@ -457,6 +462,9 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         MDG_HIP_CHECK(hipMalloc(&pl.arena, bytes));
         pl.bytes += bytes;
         char *base = (char *)pl.arena;
+        // zero digits in the slices' padding for every replay (the slicing
+        // stage writes only the valid rows and depth)
+        if (i8) MDG_HIP_CHECK(hipMemset(pl.arena, 0, bytes));
         if (i8 || res) {
             pl.flags = (int *)(base + off_flags);
             // problems without entries (empty) keep flag 0 on every replay
@@ -527,7 +535,8 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 const size_t before = pl.stages.size();
                 add_stage(kind, L, [&](int i, const BProb &b) {
                     lds = std::max(lds, blk_stage_bytes(b.S));
-                    for (int x = 0; x < (kind == 4 ? b.mp : b.np) / blk_rows_per_tile(b.S); ++x)
+                    const int rt = blk_rows_per_tile(b.S);
+                    for (int x = 0; x < ((kind == 4 ? b.m : b.n) + rt - 1) / rt; ++x)
                         for (int y = 0; y < b.kp / 128; ++y) push(i, x, y, 0);
                 });
                 if (pl.stages.size() > before) pl.stages.back().lds = lds;

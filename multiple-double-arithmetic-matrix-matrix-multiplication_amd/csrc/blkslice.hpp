@@ -52,20 +52,41 @@ __host__ __device__ __forceinline__ int blk_stage_bytes(int S, int digit_bytes =
     return blk_rows_per_tile(S, digit_bytes) * blk_row_bytes(S);
 }
 
+// Fixed-point words a slicing of S digits of w bits needs, |N| < 2^(S w - 2):
+// the smallest of three register-array sizes per limb count (the largest
+// FixedWords<L>::NW), so the conversion's word loop runs no further than N
+// reaches.  fn(std::integral_constant<int, NW>).
+template <int L, typename Fn>
+__host__ __device__ __forceinline__ void with_slice_words(int S, int w, Fn fn)
+{
+    constexpr int NWMAX = FixedWords<L>::NW;
+    constexpr int NW1 = (NWMAX + 1) / 2, NW2 = (2 * NWMAX + 2) / 3;
+    const int need = (S * w + 31) / 32;
+    if (need <= NW1)
+        fn(std::integral_constant<int, NW1>{});
+    else if (need <= NW2)
+        fn(std::integral_constant<int, NW2>{});
+    else
+        fn(std::integral_constant<int, NWMAX>{});
+}
+
 // Slices of rows r0 .. r0+RT-1 (RT = blk_rows_per_tile(S, sizeof(T))),
 // k-block kb, of an operand of `rows` valid rows and depth k (padded to
 // kp).  TRANS: X is
 // B (k x n, row stride ldx) and "row" r is column r of B; else X is A (rows
 // x k).  E[r]: the row's scale exponent, zero_scale for an all-zero row.
 // Entry with fixed-point N = floor(x 2^(S w - 2 - E)) gets its S balanced
-// digits (slices.hpp).  All 256 threads of the workgroup call this; stage:
-// blk_stage_bytes(S, sizeof(T)) bytes of LDS, 16-byte aligned.
-template <int L, bool TRANS, typename T>
+// digits (slices.hpp) in NW words (with_slice_words).  All 256 threads of
+// the workgroup call this; stage: blk_stage_bytes(S, sizeof(T)) bytes of
+// LDS, 16-byte aligned.  KEEP_PAD: the padding (rows from `rows` on, and
+// 16-byte chunks wholly past k) already holds zero digits and is not
+// written again.
+template <int L, bool TRANS, typename T, int NW, bool KEEP_PAD = false>
 __device__ __forceinline__ void slice_tile(int rows, int k, int kp, const double *__restrict__ X, long long ldx,
                                            const int *__restrict__ E, int zero_scale, int S, int w, int kb, int r0,
                                            int8_t *__restrict__ out, int8_t *stage)
 {
-    constexpr int NW = FixedWords<L>::NW;
+    static_assert(NW <= FixedWords<L>::NW, "fixed-point words");
     constexpr int EB = BND_BK / (int)sizeof(T);
     const int RT = blk_rows_per_tile(S, (int)sizeof(T));
     const int rb = blk_row_bytes(S);
@@ -95,10 +116,12 @@ __device__ __forceinline__ void slice_tile(int rows, int k, int kp, const double
     // RT x S rows of 128 bytes, eight 16-byte chunks each; chunk ch of row r
     // lands at chunk (ch ^ swz(r)) of its block row
     const int nkb = kp / EB;
+    const int kbytes = (k - kb * EB) * (int)sizeof(T);   // valid bytes of each row of this k-block
     for (int c = threadIdx.x; c < RT * S * 8; c += 256) {
         const int ch = c & 7, rs = c >> 3;
         const int rr = rs / S, t = rs - rr * S;
         const int r = r0 + rr;
+        if (KEEP_PAD && (r >= rows || ch * 16 >= kbytes)) continue;
         const int4 v = *(const int4 *)(stage + rr * rb + t * 128 + ch * 16);
         const int sw = (r & 7) ^ ((r >> 3) & 1);
         *(int4 *)(out + bnd_block(r >> 7, kb, t, S, nkb) + (r & 127) * BND_BK + (((ch ^ sw) & 7) << 4)) = v;

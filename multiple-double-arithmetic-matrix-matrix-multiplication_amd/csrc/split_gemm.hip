@@ -251,21 +251,27 @@ __global__ __launch_bounds__(256) void slice_b_kernel(int k, int n, int kp, int 
 // workgroup (blkslice.hpp).  Grid: x = row tiles (rp / RT), y = k-blocks
 // (kp / EB, EB = 128 / sizeof(T)).  TRANS: the operand is B (k x n, row
 // stride ldx) and "row" r is column r of B.
-template <int L, bool TRANS, typename T = int8_t>
+template <int L, bool TRANS, typename T, int NW>
 __global__ __launch_bounds__(256) void slice_blk_kernel(int rows, int k, int kp, const double *__restrict__ X,
                                                         long long ldx, const int *__restrict__ E, int S, int w,
                                                         int8_t *__restrict__ out)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t stage[];   // blk_stage_bytes(S, sizeof(T))
-    slice_tile<L, TRANS, T>(rows, k, kp, X, ldx, E, ZERO_SCALE, S, w, blockIdx.y,
-                            blockIdx.x * blk_rows_per_tile(S, (int)sizeof(T)), out, stage);
+    slice_tile<L, TRANS, T, NW>(rows, k, kp, X, ldx, E, ZERO_SCALE, S, w, blockIdx.y,
+                                blockIdx.x * blk_rows_per_tile(S, (int)sizeof(T)), out, stage);
 }
 
-// grid of slice_blk_kernel for rp padded rows
-template <typename T>
-static dim3 slice_blk_grid(int rp, int kp, int S)
+// slices of an rp (padded) x kp operand in the blocked layout
+template <int L, bool TRANS, typename T = int8_t>
+static void launch_slice_blk(int rows, int k, int rp, int kp, const double *X, long long ldx, const int *E, int S,
+                             int w, int8_t *out, hipStream_t s)
 {
-    return dim3((unsigned)(rp / blk_rows_per_tile(S, (int)sizeof(T))), (unsigned)(kp / (BND_BK / (int)sizeof(T))));
+    const dim3 grid((unsigned)(rp / blk_rows_per_tile(S, (int)sizeof(T))), (unsigned)(kp / (BND_BK / (int)sizeof(T))));
+    const int lds = blk_stage_bytes(S, (int)sizeof(T));
+    with_slice_words<L>(S, w, [&](auto nw) {
+        hipLaunchKernelGGL((slice_blk_kernel<L, TRANS, T, decltype(nw)::value>), grid, dim3(256), lds, s, rows, k, kp,
+                           X, ldx, E, S, w, out);
+    });
 }
 
 // --------------------------------------------------- INT8 diagonal GEMM --
@@ -1030,12 +1036,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp, 2);
             int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp, 3);
             MDG_REQUIRE(Aall && Ball, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-            hipLaunchKernelGGL((slice_blk_kernel<L, false>), slice_blk_grid<int8_t>(mpb, kp, S), dim3(256),
-                               blk_stage_bytes(S), s, m, k,
-                               kp, A, lda, E, S, w, Aall);
-            hipLaunchKernelGGL((slice_blk_kernel<L, true>), slice_blk_grid<int8_t>(npb, kp, S), dim3(256),
-                               blk_stage_bytes(S), s, n, k,
-                               kp, B, ldb, F, S, w, Ball);
+            launch_slice_blk<L, false>(m, k, mpb, kp, A, lda, E, S, w, Aall, s);
+            launch_slice_blk<L, true>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
             rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
@@ -1093,12 +1095,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
         int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp * 2, 3);
         MDG_REQUIRE(Aall && Ball && D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
-        hipLaunchKernelGGL((slice_blk_kernel<L, false, __bf16>), slice_blk_grid<__bf16>(mpb, kp, S), dim3(256),
-                           blk_stage_bytes(S, 2), s, m,
-                           k, kp, A, lda, E, S, w, Aall);
-        hipLaunchKernelGGL((slice_blk_kernel<L, true, __bf16>), slice_blk_grid<__bf16>(npb, kp, S), dim3(256),
-                           blk_stage_bytes(S, 2), s, n,
-                           k, kp, B, ldb, F, S, w, Ball);
+        launch_slice_blk<L, false, __bf16>(m, k, mpb, kp, A, lda, E, S, w, Aall, s);
+        launch_slice_blk<L, true, __bf16>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
         tm.mark(2);
         const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
         rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
