@@ -59,7 +59,8 @@ struct BProb {
     const double *A, *B;
     double *C;
     int8_t *Aall, *Ball;
-    int *D, *E, *F;
+    long long *D;   // band planes (i8band.hpp), INT64
+    int *E, *F;
     double *colsum;
     int *flag;     // the problem's flags: bit 0 non-finite input, bit 1 spread above `allow`
     int allow;     // exponent-spread allowance of the slice plan (bits)
@@ -221,9 +222,6 @@ __global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ 
 }
 
 // ------------------------------------------------------- INT8 band tiles --
-// INT32 plane count of diagonal g: its slice range cut into chunks of
-// kmax / kp slices (plane numbering of the single-problem path)
-__device__ __forceinline__ int b_chunks(int g, int kp, int kmax) { return ((g + 1) * kp + kmax - 1) / kmax; }
 
 // Entry (prob, tile row, tile column, item): one band item (i8band.hpp) of
 // one 128 x 128 tile of one problem; the item packed as g0 & 0xff | t0 << 8
@@ -254,9 +252,10 @@ __global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict_
 }
 
 // ------------------------------------------------------- reconstruction --
-// the same exact fold as the single-problem INT8 path (recon.hpp), so the
-// fused pipeline and the per-problem loop give identical bits
-template <int L, int NW, bool CHUNKED>
+// the band planes folded as in the single-problem path (recon.hpp
+// band_fold), so the fused pipeline and the per-problem loop give identical
+// bits
+template <int L, int NWB, bool CHUNKED>
 __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
     const BEnt e = T[blockIdx.x];
@@ -270,30 +269,18 @@ __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restr
     const int er = p.E[r], fc = p.F[c];
     if (er != BZERO_SCALE && fc != BZERO_SCALE) {
         const long long mn = (long long)p.m * p.n;
+        const int S = p.S, Tc = p.kmax / p.kp;
         // the planes live in global memory (a generic pointer read from the
         // problem table would make every load a flat one)
-        const __attribute__((address_space(1))) int *D = (const __attribute__((address_space(1))) int *)p.D;
-        if constexpr (!CHUNKED) {
-            diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
-                                [&](int g) { return (int)D[(long long)g * mn + idx]; }, y);
-        } else {
-            diag_fold<L, NW, 3>(p.S, p.w, er + fc + 4 - p.w * (p.S + 1),
-                                [&](int g) {
-                                    int first;
-                                    if (p.kmax % p.kp == 0) {
-                                        const int cc = p.kmax / p.kp, q = g / cc, rr = g - q * cc;
-                                        first = cc * q * (q + 1) / 2 + rr * (q + 1);
-                                    } else {
-                                        first = 0;
-                                        for (int h = 0; h < g; ++h) first += b_chunks(h, p.kp, p.kmax);
-                                    }
-                                    const int nc = b_chunks(g, p.kp, p.kmax);
-                                    long long v = 0;
-                                    for (int q = 0; q < nc; ++q) v += D[(long long)(first + q) * mn + idx];
-                                    return v;
-                                },
-                                y);
-        }
+        const __attribute__((address_space(1))) long long *D =
+            (const __attribute__((address_space(1))) long long *)p.D;
+        band_fold<L, NWB, 4, CHUNKED>(
+            S, er + fc + 4 - 8 * (S + 1),
+            [&](int q, int ch) {
+                const int plane = CHUNKED ? bnd_first_band_plane(S, 4, Tc, q) + ch : q;
+                return (long long)D[(long long)plane * mn + idx];
+            },
+            [&](int q) { return bnd_band_chunks(S, 4, Tc, q); }, y);
     }
     double *o = p.C + (r * p.ldc + c) * L;
 #pragma unroll
@@ -358,7 +345,7 @@ struct Stage {
                    // 8 vector, 9 guard (NaN-fill flagged problems)
     int L;         // limb count of the stage's problems (0: any)
     long long first, count;   // range in the entry table
-    int nw = 0;    // reconstruction: diagonal-sum words its problems need
+    int nw = 0;    // reconstruction: the largest slice count of its problems
     bool chunked = false;   // reconstruction: some problem has K-chunk planes
     int tm = 128;  // GEMM: tile rows and columns
     int g = 0;     // GEMM: band width of the band-kernel launch
@@ -449,11 +436,10 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         // pair must fit an INT32 plane (the per-problem loop handles deeper k)
         MDG_REQUIRE(b.kp <= 131071 && b.S < 256, MDG_ERROR_NOT_SUPPORTED, "problem %d too deep for the fused stages", i);
         b.kmax = (131071 / b.kp) * b.kp;
-        b.nplanes = 0;
-        for (int g = 0; g < b.S; ++g) b.nplanes += ((g + 1) * b.kp + b.kmax - 1) / b.kmax;
+        b.nplanes = bnd_band_planes(b.S, 4, b.kmax / b.kp);
         offs[i].aall = take((size_t)b.mp * b.S * b.kp);
         offs[i].ball = take((size_t)b.np * b.S * b.kp);
-        offs[i].d = take((size_t)b.nplanes * b.m * b.n * 4);
+        offs[i].d = take((size_t)b.nplanes * b.m * b.n * 8);
         offs[i].e = take((size_t)b.m * 4);
         offs[i].f = take((size_t)b.n * 4);
         offs[i].cs = take((size_t)b.n * 8);
@@ -480,7 +466,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
             if (b.S > 0) {
                 b.Aall = (int8_t *)(base + offs[i].aall);
                 b.Ball = (int8_t *)(base + offs[i].ball);
-                b.D = (int *)(base + offs[i].d);
+                b.D = (long long *)(base + offs[i].d);
             }
             b.E = (int *)(base + offs[i].e);
             b.F = (int *)(base + offs[i].f);
@@ -603,8 +589,8 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
             if (pl.stages.size() > before)
                 for (const BProb &b : pl.probs)
                     if (b.L == L && b.S) {
-                        pl.stages.back().nw = std::max(pl.stages.back().nw, diag_words_needed(b.S, b.w));
-                        pl.stages.back().chunked = pl.stages.back().chunked || b.nplanes > b.S;
+                        pl.stages.back().nw = std::max(pl.stages.back().nw, b.S);   // the largest slice count
+                        pl.stages.back().chunked = pl.stages.back().chunked || b.kmax / b.kp < b.S;
                     }
         }
     } else if (!res) {
@@ -672,12 +658,12 @@ static int launch_stage(const Plan &pl, const Stage &st, hipStream_t s)
     case 7: {
         auto rec = [&](auto lc) -> int {
             constexpr int L = decltype(lc)::value;
-            return with_words<L>(st.nw, [&](auto nw) {
-                constexpr int NW = decltype(nw)::value;
+            return with_band_words<L, 4>(st.nw, [&](auto nw) {
+                constexpr int NWB = decltype(nw)::value;
                 if (st.chunked)
-                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NW, true>), grid, blk, 0, s, P, T);
+                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NWB, true>), grid, blk, 0, s, P, T);
                 else
-                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NW, false>), grid, blk, 0, s, P, T);
+                    hipLaunchKernelGGL((b_reconstruct_kernel<L, NWB, false>), grid, blk, 0, s, P, T);
             });
         };
         int rc = (st.L == 2)   ? rec(std::integral_constant<int, 2>())

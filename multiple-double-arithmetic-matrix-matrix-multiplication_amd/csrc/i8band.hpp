@@ -21,8 +21,10 @@
 //
 // Exactness: |d_t d_u| <= 2^14, so an INT32 sum over depth K is exact while
 // K < 2^17; a band's t-range is cut into chunks of Tc slices with
-// Tc * kp <= 131071 and every (diagonal, chunk) is its own INT32 plane
-// (plane numbering of split_gemm.hip: ascending g, then chunk).
+// Tc * kp <= 131071, and each (band, chunk) writes one INT64 "band plane":
+// its G diagonal sums combined exactly, P = sum_j D_{g0+j} 2^(8 (G-1-j))
+// (|P| < 2^56), half the bytes of one INT32 plane per diagonal
+// (recon.hpp band_fold takes them apart again).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -122,12 +124,26 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
     tn = in / gsize;
 }
 
-// first INT32 plane of diagonal g when every diagonal is cut into chunks of
-// Tc slices: sum_{h<g} ceil((h+1)/Tc)
-__host__ __device__ __forceinline__ int bnd_first_plane(int g, int Tc)
+// Band planes.  Band q = 0 .. nb-1, nb = ceil(S / G), holds diagonals g0_q
+// .. g0_q + G - 1 with g0_q = S - G (nb - q) (band 0 starts below diagonal
+// 0 when G does not divide S, band_items) and has ceil((g0_q + G) / Tc)
+// K-chunks; plane (q, c) is bnd_first_band_plane(q) + c.
+__host__ __device__ __forceinline__ int bnd_band_chunks(int S, int G, int Tc, int q)
 {
-    const int q = g / Tc, r = g - q * Tc;
-    return Tc * q * (q + 1) / 2 + r * (q + 1);
+    const int nb = (S + G - 1) / G;
+    return (S - G * (nb - q) + G + Tc - 1) / Tc;
+}
+
+__host__ __device__ __forceinline__ int bnd_first_band_plane(int S, int G, int Tc, int q)
+{
+    int p = 0;
+    for (int h = 0; h < q; ++h) p += bnd_band_chunks(S, G, Tc, h);
+    return p;
+}
+
+__host__ __device__ __forceinline__ int bnd_band_planes(int S, int G, int Tc)
+{
+    return bnd_first_band_plane(S, G, Tc, (S + G - 1) / G);
 }
 
 // Band kernel.  Grid: x = output tiles (tiles_m * tiles_n), y = band items.
@@ -157,14 +173,15 @@ constexpr int bnd_lds_bytes()
 // One band item of one output tile: Atile / Btile = the tile's blocked
 // slices (block (kb, slice) at (kb * S + slice) * 16 KiB), diagonals
 // g0 .. g0+G-1, slices t of A in [t0, t1); the tile's rows start at row0 of
-// the m x n planes, its columns at col0.  lds: bnd_lds_bytes<G, PA>()
-// bytes.  IMV < IM (INT8) keeps only the first IMV 32-row MFMA tiles of
+// the m x n band planes D (INT64), its columns at col0.  lds:
+// bnd_lds_bytes<G, PA>() bytes.  IMV < IM (INT8) keeps only the first IMV 32-row MFMA tiles of
 // every wave: the work on the zero padding of a tile whose rows from
 // row0 + 32 IMV on are beyond m is left out.
 template <int G, int PA, int WM, bool BF = false, int IMV = BND_TM / WM / 32>
 __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__ Atile,
                                          const int8_t *__restrict__ Btile, int S, int nkb, int Tc, int g0, int t0,
-                                         int t1, long long row0, long long col0, int m, int n, int *__restrict__ D)
+                                         int t1, long long row0, long long col0, int m, int n,
+                                         long long *__restrict__ D)
 {
     constexpr int RA = PA + 1, RB = PA + G;
     // WM x WN waves, each TMW x TNW of every diagonal: IM x IN MFMA tiles
@@ -467,34 +484,36 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     bnd_wait_vm<0>();
 
     // acc element v of a 32 x 32 tile: row 8 (v / 4) + 4 (lane / 32) + v % 4,
-    // column lane % 32
-    const int c = t0 / Tc;
-    const long long mn = (long long)m * n;
+    // column lane % 32; the band's G sums of an entry go out as one INT64,
+    // sum_j D_j 2^(8 (G-1-j)) (a diagonal this chunk has no pair for, or one
+    // below 0, holds 0)
+    const int nbands = (S + G - 1) / G;
+    const int band = (g0 - (S - G * nbands)) / G;
+    long long *plane = D + (long long)(bnd_first_band_plane(S, G, Tc, band) + t0 / Tc) * ((long long)m * n);
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        const int g = g0 + j;
-        if (t0 > g) continue;   // this chunk adds nothing to diagonal g
-        int *plane = D + (long long)(bnd_first_plane(g, Tc) + c) * mn;
+    for (int i = 0; i < IMV; ++i)
 #pragma unroll
-        for (int i = 0; i < IMV; ++i)
+        for (int jj = 0; jj < IN; ++jj) {
+            const long long col = col0 + wn * TNW + jj * 32 + (lane & 31);
 #pragma unroll
-            for (int jj = 0; jj < IN; ++jj) {
-                const long long col = col0 + wn * TNW + jj * 32 + (lane & 31);
+            for (int v = 0; v < 16; ++v) {
+                const long long row = row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (row < m && col < n) {
+                    long long P = 0;
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const long long row =
-                        row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                    if (row < m && col < n) {
+                    for (int j = 0; j < G; ++j) {
+                        int dj;
                         if constexpr (BF)
-                            plane[row * n + col] =
-                                tot[j][i][jj][v] +
-                                ((drop && ((j * IM + i) * IN + jj) / KT == NB - 1) ? 0 : (int)acc[j][i][jj][v]);
+                            dj = tot[j][i][jj][v] +
+                                 ((drop && ((j * IM + i) * IN + jj) / KT == NB - 1) ? 0 : (int)acc[j][i][jj][v]);
                         else
-                            plane[row * n + col] = acc[j][i][jj][v];
+                            dj = acc[j][i][jj][v];
+                        P = P * 256 + dj;
                     }
+                    plane[row * n + col] = P;
                 }
             }
-    }
+        }
 }
 
 
@@ -502,7 +521,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 template <int G, int PA, int WM>
 __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                                                       int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
-                                                      BandTable tab, int *__restrict__ D)
+                                                      BandTable tab, long long *__restrict__ D)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
     const BandItem it = tab.item[blockIdx.y];
@@ -517,7 +536,7 @@ __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__
 template <int G, int PA, int WM>
 __global__ __launch_bounds__(256) void band_bf16_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                                                         int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
-                                                        BandTable tab, int *__restrict__ D)
+                                                        BandTable tab, long long *__restrict__ D)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
     const BandItem it = tab.item[blockIdx.y];

@@ -59,6 +59,55 @@ __device__ __forceinline__ void top_pieces(const double (&d)[NP], int tp, double
     for (int s = 0; s < K; ++s) out[s] = (s < NP) ? r[s] : 0.0;
 }
 
+// The fold of |Z| 2^(w pad), given as NW words of ws = w GS bits (z[0]
+// least significant), into L limbs, negated when neg: pieces of two words
+// (2 ws <= 52 bits, exact) from the leading nonzero one, the first L + 1
+// through fast_renorm, the rest through inc_d; sc0 is the scale exponent of
+// Z's unit.  Any exact way of forming the words gives the same limbs.
+template <int L, int NW, int GS>
+__device__ __forceinline__ void fold_words(const long long (&z)[NW], bool neg, int w, int pad, int sc0, double (&y)[L])
+{
+    const int ws = w * GS;
+    // pieces of two words (2 ws <= 52 bits, exact), the leading one first:
+    // K pieces carry at least 1 + 2 ws (K - 1) >= 53 L + 20 bits; the first
+    // L + 1 go through fast_renorm, the others one at a time through inc_d
+    constexpr int NP = (NW + 1) / 2;
+    constexpr int WSMIN = GS == 3 ? 24 : (GS == 2 ? 16 : 14);
+    constexpr int KMAX = (53 * L + 19) / (2 * WSMIN) + 2;
+    const int K = (53 * L + 19) / (2 * ws) + 2;
+    double d[NP];
+    int tp = -1;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const long long hi = (2 * p + 1 < NW) ? z[2 * p + 1] : 0;
+        d[p] = fma((double)hi, ldexp(1.0, ws), (double)z[2 * p]);
+        tp = (d[p] != 0.0) ? p : tp;
+    }
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    if (tp < 0) return;
+    const int sc = sc0 - w * pad;
+    double top[KMAX];
+    top_pieces<NP, KMAX>(d, tp, top);
+    auto piece = [&](int s) { return ldexp(top[s], 2 * ws * (tp - s) + sc); };
+    {
+        double f[L + 1];
+#pragma unroll
+        for (int s = 0; s <= L; ++s) f[s] = (s < K) ? piece(s) : 0.0;
+        fast_renorm<L>(f, y);
+    }
+#pragma unroll
+    for (int s = L + 1; s < KMAX; ++s) {
+        if (s >= K) break;
+        const double v = piece(s);
+        if (v != 0.0) md_inc_d<L>(y, v);
+    }
+    if (neg) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) y[l] = -y[l];
+    }
+}
+
 // D(g) returns the diagonal sum g (0 <= g < S; it is called with every index
 // of the word layout, clamped into range, so that all loads issue as
 // straight-line code) as an INT64, or as an INT32 when every sum fits one
@@ -120,44 +169,119 @@ __device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, doubl
             cy = v >> ws;
         }
     }
-    // pieces of two words (2 ws <= 52 bits, exact), the leading one first:
-    // K pieces carry at least 1 + 2 ws (K - 1) >= 53 L + 20 bits; the first
-    // L + 1 go through fast_renorm, the others one at a time through inc_d
-    constexpr int NP = (NW + 1) / 2;
-    constexpr int WSMIN = GS == 3 ? 24 : (GS == 2 ? 16 : 14);
-    constexpr int KMAX = (53 * L + 19) / (2 * WSMIN) + 2;
-    const int K = (53 * L + 19) / (2 * ws) + 2;
-    double d[NP];
-    int tp = -1;
+    fold_words<L, NW, GS>(z, neg, w, pad, sc0, y);
+}
+
+// Band planes (i8band.hpp): per band q of G diagonals (first diagonal
+// g0_q = S - G (nb - q), nb = ceil(S / G)) and K-chunk c, the exact INT64
+// P = sum_j D_{g0_q + j} 2^(w (G-1-j)), so that Z = sum_g D_g 2^(w (S-1-g))
+// = sum_q (sum_c P_qc) 2^(w G (nb-1-q)): band q lands whole on word nb-1-q of
+// a layout of wb = w G-bit words (32 for INT8 bands of 4, 24 for BF16 bands
+// of 3).  P(q, c) returns the plane value, nch(q) the band's chunk count
+// (CHUNKED false: one chunk each, P < 2^(31 + w (G-1)) added whole; true:
+// each P split at wb bits).  The words are carried into two's complement,
+// the magnitude taken, and re-cut into the 24-bit words of Z 2^(8 pad),
+// pad = 3 ceil(S/3) - S, which fold_words takes for w = 8 -- the same
+// pieces, hence the same limbs, as diag_fold over the diagonal planes.
+template <int L, int NWB, int G, bool CHUNKED, typename PlaneFn, typename ChunkFn>
+__device__ __forceinline__ void band_fold(int S, int sc0, PlaneFn P, ChunkFn nch, double (&y)[L])
+{
+    constexpr int w = 8, WB = w * G;
+    static_assert(WB == 24 || WB == 32, "band words of 24 or 32 bits");
+    constexpr int NW24 = (NWB * WB + 16 + 23) / 24;   // 24-bit words of |Z| 2^(8 pad)
+    const int nb = (S + G - 1) / G;
+    long long z[NWB];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-        const long long hi = (2 * p + 1 < NW) ? z[2 * p + 1] : 0;
-        d[p] = fma((double)hi, ldexp(1.0, ws), (double)z[2 * p]);
-        tp = (d[p] != 0.0) ? p : tp;
+    for (int i = 0; i < NWB; ++i) z[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NWB; ++i) {
+        const int q = nb - 1 - i;
+        if constexpr (!CHUNKED) {
+            // unconditional load (clamped band) and a mask, not a branch
+            z[i] += P(max(q, 0), 0) & -(long long)(q >= 0);
+        } else {
+            if (q < 0) break;   // uniform: the same S on every lane
+            const int nc = nch(q);
+            for (int c = 0; c < nc; ++c) {
+                const long long v = P(q, c);
+                z[i] += v & ((1ll << WB) - 1);
+                if (i + 1 < NWB) z[i + 1] += v >> WB;
+            }
+        }
+        if (i % RECON_BATCH == RECON_BATCH - 1) __builtin_amdgcn_sched_barrier(0);
     }
+    const long long mask = (1ll << WB) - 1;
+    long long carry = 0;
 #pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = 0.0;
-    if (tp < 0) return;
-    const int sc = sc0 - w * pad;
-    double top[KMAX];
-    top_pieces<NP, KMAX>(d, tp, top);
-    auto piece = [&](int s) { return ldexp(top[s], 2 * ws * (tp - s) + sc); };
-    {
-        double f[L + 1];
-#pragma unroll
-        for (int s = 0; s <= L; ++s) f[s] = (s < K) ? piece(s) : 0.0;
-        fast_renorm<L>(f, y);
+    for (int i = 0; i < NWB; ++i) {
+        const long long v = z[i] + carry;
+        z[i] = v & mask;
+        carry = v >> WB;
     }
-#pragma unroll
-    for (int s = L + 1; s < KMAX; ++s) {
-        if (s >= K) break;
-        const double v = piece(s);
-        if (v != 0.0) md_inc_d<L>(y, v);
-    }
+    const bool neg = carry < 0;
     if (neg) {
+        long long cy = 1;
 #pragma unroll
-        for (int l = 0; l < L; ++l) y[l] = -y[l];
+        for (int i = 0; i < NWB; ++i) {
+            const long long v = (mask - z[i]) + cy;
+            z[i] = v & mask;
+            cy = v >> WB;
+        }
     }
+    // |Z| 2^(8 pad) in 24-bit words: shift the WB-bit words left by sh bits
+    // (sh < WB), then cut the bit string at multiples of 24
+    const int pad = 3 * ((S + 2) / 3) - S, sh = 8 * pad;
+    uint32_t u[NWB + 1];
+#pragma unroll
+    for (int i = 0; i <= NWB; ++i) {
+        const uint32_t cur = (i < NWB) ? (uint32_t)z[i] : 0u, prev = (i > 0) ? (uint32_t)z[i - 1] : 0u;
+        if constexpr (WB == 32)
+            u[i] = sh ? ((cur << sh) | (prev >> (32 - sh))) : cur;
+        else
+            u[i] = ((cur << sh) | (prev >> (24 - sh))) & 0xffffffu;
+    }
+    long long z24[NW24];
+#pragma unroll
+    for (int j = 0; j < NW24; ++j) {
+        const int bit = 24 * j;   // compile-time after unrolling
+        const int wi = bit / WB, off = bit % WB;
+        const uint32_t lo = (wi <= NWB) ? u[wi] : 0u;
+        const uint32_t hi = (wi + 1 <= NWB) ? u[wi + 1] : 0u;
+        const uint64_t both = (uint64_t)lo | ((uint64_t)hi << WB);
+        z24[j] = (long long)((both >> off) & 0xffffffu);
+    }
+    fold_words<L, NW24, 3>(z24, neg, w, pad, sc0, y);
+}
+
+// band-fold word count NWB (WB = 8 G bits) for S slices: the bands, then
+// room for the top band's 64-bit sums and the sign; fn is called with
+// std::integral_constant<int, NWB> of the smallest fitting instantiation
+template <int L, int G, typename Fn>
+inline int with_band_words(int S, Fn fn)
+{
+    const int need = (S + G - 1) / G + (64 + 8 * G - 1) / (8 * G) + 1;
+    auto pick = [&](auto nw) -> bool {
+        if (decltype(nw)::value < need) return false;
+        fn(nw);
+        return true;
+    };
+    using std::integral_constant;
+    bool ok;
+    if constexpr (L == 2)
+        ok = pick(integral_constant<int, 8>()) || pick(integral_constant<int, 12>()) ||
+             pick(integral_constant<int, 16>());
+    else if constexpr (L == 4)
+        ok = pick(integral_constant<int, 12>()) || pick(integral_constant<int, 16>()) ||
+             pick(integral_constant<int, 24>());
+    else if constexpr (L == 8)
+        ok = pick(integral_constant<int, 20>()) || pick(integral_constant<int, 28>()) ||
+             pick(integral_constant<int, 40>());
+    else
+        ok = pick(integral_constant<int, 40>()) || pick(integral_constant<int, 56>()) ||
+             pick(integral_constant<int, 92>());
+    if (ok) return MDG_SUCCESS;
+    set_error("no band reconstruction for %d slices at L=%d", S, L);
+    return MDG_ERROR_RANGE;
 }
 
 // smallest word-count instantiation with at least `need` words; fn is

@@ -753,6 +753,53 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
     for (int l = 0; l < L; ++l) o[l] = y[l];
 }
 
+// Band planes of the band kernels (i8band.hpp, INT8 bands of G = 4 or BF16
+// bands of 3; w = 8): one INT64 plane per (band, K-chunk), unchunked when
+// one chunk holds every slice (Tc >= S)
+template <int L, int NWB, int G, bool CHUNKED>
+__global__ __launch_bounds__(256) void reconstruct_band_kernel(int m, int n, int S, int Tc,
+                                                               const long long *__restrict__ D,
+                                                               const int *__restrict__ E, const int *__restrict__ F,
+                                                               double *__restrict__ C, long long ldc)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * n) return;
+    const long long r = idx / n, c = idx - r * n;
+    double y[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = 0.0;
+    const int er = E[r], fc = F[c];
+    if (er != ZERO_SCALE && fc != ZERO_SCALE) {
+        const long long mn = (long long)m * n;
+        band_fold<L, NWB, G, CHUNKED>(
+            S, er + fc + 4 - 8 * (S + 1),
+            [&](int q, int ch) {
+                const int plane = CHUNKED ? bnd_first_band_plane(S, G, Tc, q) + ch : q;
+                return D[(long long)plane * mn + idx];
+            },
+            [&](int q) { return bnd_band_chunks(S, G, Tc, q); }, y);
+    }
+    double *o = C + (r * ldc + c) * L;
+#pragma unroll
+    for (int l = 0; l < L; ++l) o[l] = y[l];
+}
+
+template <int L, int G>
+static int launch_band_recon(int m, int n, int S, int Tc, const long long *D, const int *E, const int *F, double *C,
+                             long long ldc, hipStream_t s)
+{
+    const dim3 grid((unsigned)(((long long)m * n + 255) / 256));
+    return with_band_words<L, G>(S, [&](auto nwb) {
+        constexpr int NWB = decltype(nwb)::value;
+        if (Tc < S)   // some band has more than one K-chunk
+            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, true>), grid, dim3(256), 0, s, m, n, S, Tc, D, E,
+                               F, C, ldc);
+        else
+            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, false>), grid, dim3(256), 0, s, m, n, S, Tc, D, E,
+                               F, C, ldc);
+    });
+}
+
 // ------------------------------------------------------- rocBLAS backend --
 // The diagonal GEMMs are plain INT8 x INT8 -> INT32 GEMMs; rocBLAS runs each
 // K-chunk plane as one gemm_ex call (column-major view: D_p^T (n x m) =
@@ -800,7 +847,7 @@ static bool build_band_table(int S, int G, int Tc, BandTable &tab)
 // one launch of the band kernel over the problem's table: bands of 4
 // diagonals on INT8 slices
 static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
-                       int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
+                       int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
@@ -812,7 +859,7 @@ static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *B
 
 // the same over blocked BF16 slices (nkb = kp / 64), bands of 3 diagonals
 static int launch_band_bf16(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
-                            int tiles_m, int tiles_n, int m, int n, int *D, hipStream_t s)
+                            int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
@@ -1028,7 +1075,10 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
         BandTable tab;
         if (engine == I8E_BAND && (kp > 131071 || !build_band_table(S, 4, kmax / kp, tab))) engine = I8E_TILE256;
-        int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
+        // the band kernel writes one INT64 plane per (band, chunk), the
+        // other engines one INT32 plane per (diagonal, chunk)
+        const int nbp = (engine == I8E_BAND) ? bnd_band_planes(S, 4, kmax / kp) : 0;
+        void *D = scratch(s, (engine == I8E_BAND) ? (size_t)nbp * mn * 8 : (size_t)nplanes * mn * 4, 4);
         MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
         if (engine == I8E_BAND) {
             // blocked slices, 128-row tiles
@@ -1040,7 +1090,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             launch_slice_blk<L, true>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-            rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+            rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, (long long *)D, s);
             if (rc) return rc;
         } else {
             // row-major K-contiguous slices: A_all[r][t kp + kk], Bt_all[c][(S-1-u) kp + kk]
@@ -1054,33 +1104,36 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                                dim3(256), 0, s, k, n, kp, np8, B, ldb, F, S, w, Ball);
             tm.mark(2);
             if (engine == I8E_LIB) {
-                rc = i8_planes_library(Aall, Ball, S, kp, kmax, m, n, D, s, &lib_engine_used);
+                rc = i8_planes_library(Aall, Ball, S, kp, kmax, m, n, (int *)D, s, &lib_engine_used);
                 if (rc) return rc;
             } else {
                 MDG_REQUIRE(nplanes <= 65535, MDG_ERROR_RANGE, "too many INT8 planes (%d)", nplanes);
                 const int tiles_m8 = mp8 / 256, tiles_n8 = np8 / 256;
                 dim3 grid(tiles_m8 * tiles_n8, 1, (unsigned)nplanes);
                 hipLaunchKernelGGL(diag_i8_kernel, grid, dim3(256), 0, s, Aall, Ball, S, kp, kmax, m, n, tiles_m8,
-                                   tiles_n8, D);
+                                   tiles_n8, (int *)D);
                 MDG_LAUNCH_CHECK("diag_i8_kernel");
             }
         }
         p.backend = engine;
         tm.mark(3);
-        rc = with_diag_words<L>(S, w, [&](auto nw) {
-            constexpr int NW = decltype(nw)::value;
-            if (nplanes > S)
-                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, true>), dim3(blocks_for<int>(mn, 256)), dim3(256),
-                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
-            else
-                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, false>), dim3(blocks_for<int>(mn, 256)), dim3(256),
-                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
-        });
+        if (engine == I8E_BAND)
+            rc = launch_band_recon<L, 4>(m, n, S, kmax / kp, (const long long *)D, E, F, C, ldc, s);
+        else
+            rc = with_diag_words<L>(S, w, [&](auto nw) {
+                constexpr int NW = decltype(nw)::value;
+                if (nplanes > S)
+                    hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, true>), dim3(blocks_for<int>(mn, 256)),
+                                       dim3(256), 0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+                else
+                    hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, false>), dim3(blocks_for<int>(mn, 256)),
+                                       dim3(256), 0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
+            });
         if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071) {
-        // band kernel over blocked BF16 slices: the INT32 chunk planes of the
-        // INT8 path (same integer sums, FP32 partial sums moved into INT32
-        // every 1024 products), so the INT8 reconstruction applies unchanged
+        // band kernel over blocked BF16 slices: the same exact integer sums
+        // as the INT8 path (FP32 partial sums moved into INT32 every 1024
+        // products), as INT64 band planes of 3 diagonals
         const int kmax = (131071 / kp) * kp;
         int nplanes = 0;
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
@@ -1090,7 +1143,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         BandTable tab;
         const bool fits = build_band_table(S, 3, kmax / kp, tab);
         MDG_REQUIRE(fits, MDG_ERROR_RANGE, "too many BF16 band items for %d slices", S);
-        int *D = (int *)scratch(s, (size_t)nplanes * mn * 4, 4);
+        long long *D = (long long *)scratch(s, (size_t)bnd_band_planes(S, 3, kmax / kp) * mn * 8, 4);
         const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
         int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp * 2, 3);
@@ -1103,15 +1156,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         if (rc) return rc;
         p.backend = I8E_BAND;
         tm.mark(3);
-        rc = with_diag_words<L>(S, w, [&](auto nw) {
-            constexpr int NW = decltype(nw)::value;
-            if (nplanes > S)
-                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, true>), dim3(blocks_for<int>(mn, 256)), dim3(256),
-                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
-            else
-                hipLaunchKernelGGL((reconstruct_i8_kernel<L, NW, false>), dim3(blocks_for<int>(mn, 256)), dim3(256),
-                                   0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
-        });
+        rc = launch_band_recon<L, 3>(m, n, S, kmax / kp, D, E, F, C, ldc, s);
         if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
         // depth beyond one INT32 chunk slice: INT64 diagonal planes

@@ -62,6 +62,41 @@ def test_batched_int8_persistent_equals_loop(mdg, orc, torch_cuda):
             assert_bits_equal(to_host(p["C"]), C, f"int8 batch rep={rep} problem {i}")
 
 
+def test_batched_int8_edge_shapes(mdg, orc, torch_cuda):
+    """Row counts that end a 128-row tile in each of its four 32-row MFMA
+    tiles (the band kernel leaves out the rows past m), depths that end
+    anywhere in a 128-byte k-block (the slicing stage stores only the chunks
+    that hold digits of k; the plan zeroed the rest once), one-row and
+    one-column problems: the persistent pipeline, built and replayed on new
+    data, is bit-identical to the loop of per-problem calls (the single-problem
+    kernels compute every padded row) and within the exact bound."""
+    shapes = [(1, 1, 1), (32, 5, 15), (33, 40, 127), (64, 1, 129), (65, 33, 300), (96, 128, 7),
+              (97, 129, 200), (128, 64, 128), (129, 17, 33), (200, 96, 257)]
+    rng = np.random.default_rng(21)
+    probs, host = [], []
+    for i, (m, n, k) in enumerate(shapes):
+        L = (2, 4)[i % 2]
+        A = orc.generate(orc.GEN_QD, L, m, k, 2100 + 2 * i, -8, 8)
+        B = orc.generate(orc.GEN_QD, L, k, n, 2101 + 2 * i, -8, 8)
+        probs.append({"L": L, "m": m, "n": n, "k": k, "A": to_dev(A), "B": to_dev(B),
+                      "C": to_dev(np.full(m * n * L, np.nan))})
+        host.append((L, m, n, k, A, B))
+    for rep in range(3):           # build + capture, then replays on new data
+        if rep:
+            _refill(probs, host, orc, int(rng.integers(1, 1 << 30)))
+        mdg.gemm_batched(probs, method="int8", persistent=0, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        loop = [to_host(p["C"]).copy() for p in probs]
+        for p in probs:
+            p["C"].fill_(np.nan)
+        mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        for i, (p, C, (L, m, n, k, A, B)) in enumerate(zip(probs, loop, host)):
+            assert_bits_equal(to_host(p["C"]), C, f"int8 edge batch rep={rep} problem {i} {m}x{n}x{k}")
+            err, _ = orc.normwise_error(L, m, n, k, A, B, C)
+            assert err <= 4.0 * k * 2.0 ** (-53 * L), (rep, L, m, n, k, err)
+
+
 def test_batched_int8_graph_replay_sees_new_data(mdg, orc, torch_cuda):
     probs, host = _batch(mdg, orc, 12, 7)
     mdg.gemm_batched(probs, method="int8", persistent=1, spread_bits=16)   # build
