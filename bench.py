@@ -379,21 +379,13 @@ def main():
         work = P * 2.0 * m * n * k
         peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
         kernel = "diag_f64_kernel"
-    # launches of the dominant kernel per step.  The band kernels run one
-    # launch over the full bands and, when the slice count is not a multiple
-    # of the band width, one short launch over the narrow band on the first
-    # diagonals (INT8 at qd: diagonal 0, 1 of 435 slice pairs, 0.1 ms); the
-    # GEMM stage's time is reported as the full-band launch's, which it
-    # exceeds by that short launch.  Library engines: one GEMM per INT32
-    # plane; other kernels: one launch covering all diagonals.
-    narrow = None
+    # launches of the dominant kernel per step: the band kernels run one
+    # launch over every band item (the leftover diagonals as a shifted band
+    # in the same launch); library engines one GEMM per INT32 plane; other
+    # kernels one launch covering all diagonals
     if head["engine"] in ("band_i8_kernel", "band_bf16_kernel"):
         kernel = head["engine"] + ("<4,2,1>" if head["engine"] == "band_i8_kernel" else "<3,2,1>")
-        gband = 4 if head["engine"] == "band_i8_kernel" else 3
         launches = 1
-        if head["slices"] % gband:
-            narrow = f"the stage also runs {head['engine']}<{head['slices'] % gband},2,1> over diagonals " \
-                     f"0..{head['slices'] % gband - 1}"
     elif head_path == "int8" and head["engine"] in ("rocblas", "hipblaslt"):
         kp = -(-k // 128) * 128
         kmax = (131071 // kp) * kp if kp <= 131071 else 130944
@@ -408,8 +400,7 @@ def main():
                 "launches_per_step": launches, "work_per_launch": work / launches,
                 "avg_launch_ms": gemm_ms / launches,
                 "note": "achieved = algorithmic ops (P*2mnk) / diagonal-GEMM time from HIP events "
-                        "on the launch stream; INT8 ops count as 2 per multiply-add"
-                        + (f"; {narrow}" if narrow else "")}
+                        "on the launch stream; INT8 ops count as 2 per multiply-add"}
 
     cpu = None
     if not a.skip_cpu:
