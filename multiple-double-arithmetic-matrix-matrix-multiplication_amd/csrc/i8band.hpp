@@ -490,14 +490,17 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     const int nbands = (S + G - 1) / G;
     const int band = (g0 - (S - G * nbands)) / G;
     long long *plane = D + (long long)(bnd_first_band_plane(S, G, Tc, band) + t0 / Tc) * ((long long)m * n);
+    // the lane index again from the exec mask, so that no lane-dependent
+    // value has to live (in scratch memory) across the main loop
+    const int el = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 #pragma unroll
     for (int i = 0; i < IMV; ++i)
 #pragma unroll
         for (int jj = 0; jj < IN; ++jj) {
-            const long long col = col0 + wn * TNW + jj * 32 + (lane & 31);
+            const long long col = col0 + wn * TNW + jj * 32 + (el & 31);
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const long long row = row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                const long long row = row0 + wm * TMW + i * 32 + (v >> 2) * 8 + (el >> 5) * 4 + (v & 3);
                 if (row < m && col < n) {
                     long long P = 0;
 #pragma unroll
