@@ -33,7 +33,7 @@ struct FixedWords {
 // and e = biased exponent - 1075; subnormal: the raw 52 bits and e = -1074)
 __device__ __forceinline__ void limb_decomp(double x, long long &m, int &e)
 {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
     const int ex = (int)((u >> 52) & 0x7ff);
     const long long mant = (long long)((u & ((1ull << 52) - 1)) | ((ex != 0) ? (1ull << 52) : 0ull));
     e = ((ex != 0) ? ex : 1) - 1075;

@@ -1,0 +1,156 @@
+// host_fixed_fold.hip -- the fixed-point conversion, digit extraction and
+// reconstruction folds of the slice GEMMs (csrc/slices.hpp, csrc/recon.hpp),
+// compiled for the host so that tests/test_host_fixed_fold.py can check them
+// without a GPU.
+//
+//   host_fixed_fold fixed < cases   per line "L shift S w x_0 .. x_{L-1}"
+//                                   (limbs as hexadecimal bit patterns):
+//                                   prints N's NW words (least significant
+//                                   first), then the S digits of
+//                                   fixed_digits and, for w = 8, those of
+//                                   fixed_digits8
+//   host_fixed_fold fold SEED N     N random cases of diagonal sums: limbs
+//                                   of band_fold over band planes (bands of
+//                                   4 and of 3, chunked and unchunked) and
+//                                   of diag_fold over the same sums; prints
+//                                   "mismatch ..." lines and "cases C"
+#include <hip/hip_runtime.h>
+#undef __device__
+#define __device__ __attribute__((host, device))
+#define __builtin_amdgcn_sched_barrier(x) ((void)0)
+#include "../../multiple-double-arithmetic-matrix-matrix-multiplication_amd/csrc/i8band.hpp"
+#include "../../multiple-double-arithmetic-matrix-matrix-multiplication_amd/csrc/recon.hpp"
+#include "../../multiple-double-arithmetic-matrix-matrix-multiplication_amd/csrc/slices.hpp"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace mdg {
+void set_error(const char *, ...) {}
+}
+using namespace mdg;
+
+template <int L, int NW>
+static void fixed_case(int shift, int S, int w, const double *x)
+{
+    uint32_t N[NW];
+    to_fixed<L, NW>(x, shift, N);
+    for (int j = 0; j < NW; ++j) std::printf("%08x ", N[j]);
+    std::printf("|");
+    std::vector<int> d(S, 0);
+    fixed_digits<NW>(N, S, w, [&](int t, int v) { d[t] = v; });
+    for (int t = 0; t < S; ++t) std::printf(" %d", d[t]);
+    if (w == 8) {
+        std::printf(" |");
+        uint32_t M[NW];
+        std::memcpy(M, N, sizeof(M));
+        fixed_digits8<NW>(M, S, [&](int t, int v) { d[t] = v; });
+        for (int t = 0; t < S; ++t) std::printf(" %d", d[t]);
+    }
+    std::printf("\n");
+}
+
+static int run_fixed()
+{
+    char line[4096];
+    while (std::fgets(line, sizeof(line), stdin)) {
+        char *p = line;
+        const int L = (int)std::strtol(p, &p, 10), shift = (int)std::strtol(p, &p, 10);
+        const int S = (int)std::strtol(p, &p, 10), w = (int)std::strtol(p, &p, 10);
+        double x[16];
+        for (int l = 0; l < L; ++l) {
+            const unsigned long long b = std::strtoull(p, &p, 16);
+            std::memcpy(&x[l], &b, 8);
+        }
+        switch (L) {
+        case 2: fixed_case<2, FixedWords<2>::NW>(shift, S, w, x); break;
+        case 4: fixed_case<4, FixedWords<4>::NW>(shift, S, w, x); break;
+        case 8: fixed_case<8, FixedWords<8>::NW>(shift, S, w, x); break;
+        default: fixed_case<16, FixedWords<16>::NW>(shift, S, w, x); break;
+        }
+    }
+    return 0;
+}
+
+// one case: random chunk sums for S slices, chunks of Tc slices, bands of G
+template <int L, int G>
+static bool fold_case(std::mt19937_64 &rng, int S, int Tc, bool big)
+{
+    const int nb = (S + G - 1) / G;
+    std::vector<long long> Dg(S, 0), Pl(bnd_band_planes(S, G, Tc), 0);
+    for (int q = 0; q < nb; ++q) {
+        const int g0 = S - G * (nb - q);
+        for (int c = 0; c < bnd_band_chunks(S, G, Tc, q); ++c) {
+            long long P = 0;
+            for (int j = 0; j < G; ++j) {
+                const int g = g0 + j;
+                long long d = 0;
+                if (g >= 0 && c * Tc <= g) {
+                    const long long lim = big ? (1ll << 31) - 1 : (1ll << 20);
+                    d = (long long)(rng() % (2 * lim + 1)) - lim;
+                    if (rng() % 9 == 0) d = 0;
+                    if (rng() % 13 == 0) d = (rng() & 1) ? lim : -lim;
+                }
+                if (g >= 0) Dg[g] += d;
+                P = P * 256 + d;
+            }
+            Pl[bnd_first_band_plane(S, G, Tc, q) + c] = P;
+        }
+    }
+    const int sc0 = (int)(rng() % 200) - 100 - 8 * (S + 1);
+    double y1[L], y2[L];
+    const int rc1 = with_band_words<L, G>(S, [&](auto nwb) {
+        constexpr int NWB = decltype(nwb)::value;
+        auto P = [&](int q, int c) { return Pl[bnd_first_band_plane(S, G, Tc, q) + c]; };
+        auto nch = [&](int q) { return bnd_band_chunks(S, G, Tc, q); };
+        if (Tc < S)
+            band_fold<L, NWB, G, true>(S, sc0, P, nch, y1);
+        else
+            band_fold<L, NWB, G, false>(S, sc0, P, nch, y1);
+    });
+    const int rc2 = with_diag_words<L>(S, 8, [&](auto nw) {
+        constexpr int NW = decltype(nw)::value;
+        diag_fold<L, NW, 3>(S, 8, sc0, [&](int g) { return Dg[g]; }, y2);
+    });
+    if (rc1 || rc2 || std::memcmp(y1, y2, sizeof(y1))) {
+        std::printf("mismatch L=%d G=%d S=%d Tc=%d rc=%d,%d: %a %a\n", L, G, S, Tc, rc1, rc2, y1[0], y2[0]);
+        return false;
+    }
+    return true;
+}
+
+static int run_fold(unsigned long long seed, int n)
+{
+    std::mt19937_64 rng(seed);
+    int cases = 0, bad = 0;
+    for (int it = 0; it < n; ++it) {
+        const int Tc = (it % 3 == 0) ? 1 + (int)(rng() % 5) : 1000;
+        const bool big = it & 1;
+        bad += !fold_case<2, 4>(rng, 1 + (int)(rng() % 31), Tc, big);
+        bad += !fold_case<4, 4>(rng, 1 + (int)(rng() % 47), Tc, big);
+        bad += !fold_case<4, 3>(rng, 1 + (int)(rng() % 47), Tc, big);
+        bad += !fold_case<8, 4>(rng, 1 + (int)(rng() % 79), Tc, big);
+        cases += 4;
+        if (it % 8 == 0) {
+            bad += !fold_case<16, 4>(rng, 1 + (int)(rng() % 143), Tc, big);
+            bad += !fold_case<16, 3>(rng, 1 + (int)(rng() % 143), Tc, big);
+            cases += 2;
+        }
+        if (bad > 10) break;
+    }
+    std::printf("cases %d\n", cases);
+    return bad ? 1 : 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc >= 2 && std::string(argv[1]) == "fixed") return run_fixed();
+    if (argc >= 4 && std::string(argv[1]) == "fold") return run_fold(std::strtoull(argv[2], nullptr, 10), std::atoi(argv[3]));
+    std::fprintf(stderr, "usage: %s fixed < cases | fold SEED N\n", argv[0]);
+    return 2;
+}
