@@ -189,6 +189,32 @@ def test_int8_band_kernel_shapes(mdg, orc, torch_cuda, L, m, n, k, emax):
     assert err <= _bound(L, k), (err, i2)
 
 
+@pytest.mark.parametrize("L,m,n,k", [(4, 150, 140, 160), (2, 40, 33, 70000)])
+def test_band_kernel_every_leftover_band(mdg, orc, torch_cuda, L, m, n, k):
+    # slice counts S with every S mod 4 (INT8 bands) and S mod 3 (BF16
+    # bands): the leftover diagonals run as a band shifted below diagonal 0;
+    # at k = 70000 (one slice per K-chunk) that band is itself cut into
+    # chunks.  Band kernels against the 256 x 256 single-diagonal kernel,
+    # bit for bit.
+    A = orc.generate(orc.GEN_QD, L, m, k, 41 + k, -6, 6)
+    B = orc.generate(orc.GEN_QD, L, k, n, 42 + k, -6, 6)
+    dA, dB = to_dev(A), to_dev(B)
+    seen = set()
+    for tb in range(53 * L - 40, 53 * L + 16, 4):
+        C2, i2 = gemm(mdg, L, m, n, k, dA, dB, "int8", target_bits=tb)
+        S = i2["slices"]
+        if S in seen:
+            continue
+        seen.add(S)
+        print(f"L={L} {m}x{n}x{k} target {tb} S={S}", flush=True)
+        C4, i4 = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256", target_bits=tb)
+        C5, i5 = gemm(mdg, L, m, n, k, dA, dB, "bf16", target_bits=tb)
+        assert i2["backend"] == 2 and i4["backend"] == 4 and i4["slices"] == S == i5["slices"], (i2, i4, i5)
+        assert np.array_equal(to_host(C2).view(np.uint64), to_host(C4).view(np.uint64)), S
+        assert np.array_equal(to_host(C2).view(np.uint64), to_host(C5).view(np.uint64)), S
+    assert {S % 4 for S in seen} == {0, 1, 2, 3} and {S % 3 for S in seen} == {0, 1, 2}, sorted(seen)
+
+
 def _digits_127(L, rows, cols):
     # every entry 1/2 + sum_{t=1}^{6L-1} 127 * 2^(-8t-6), limb l holding the
     # terms t = 6l .. 6l+5 (at most 48 bits, exact): the row scale is 2^0,
