@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "md_arith.hpp"
@@ -172,6 +173,28 @@ __device__ __forceinline__ void diag_fold(int S, int w, int sc0, DiagFn D, doubl
     fold_words<L, NW, GS>(z, neg, w, pad, sc0, y);
 }
 
+// bits [24 J, 24 J + 24) of the bit string held in words of WB bits, u[0]
+// least significant (J, and so every index, a compile-time constant)
+template <int NU, int WB, int J>
+__device__ __forceinline__ long long bits24_at(const uint32_t (&u)[NU])
+{
+    constexpr int bit = 24 * J, wi = bit / WB, off = bit % WB;
+    const uint32_t lo = (wi < NU) ? u[wi] : 0u;
+    const uint32_t hi = (wi + 1 < NU) ? u[wi + 1] : 0u;
+    const uint64_t both = (uint64_t)lo | ((uint64_t)hi << WB);
+    return (long long)((both >> off) & 0xffffffu);
+}
+
+// z[J] = bits24_at<J>(u) for every J, expanded at compile time (a loop
+// this long is not always unrolled, and a run-time index would put u in
+// scratch memory)
+template <int NU, int WB, int... J>
+__device__ __forceinline__ void recut24(const uint32_t (&u)[NU], long long (&z)[sizeof...(J)],
+                                        std::integer_sequence<int, J...>)
+{
+    ((z[J] = bits24_at<NU, WB, J>(u)), ...);
+}
+
 // Band planes (i8band.hpp): per band q of G diagonals (first diagonal
 // g0_q = S - G (nb - q), nb = ceil(S / G)) and K-chunk c, the exact INT64
 // P = sum_j D_{g0_q + j} 2^(w (G-1-j)), so that Z = sum_g D_g 2^(w (S-1-g))
@@ -241,15 +264,7 @@ __device__ __forceinline__ void band_fold(int S, int sc0, PlaneFn P, ChunkFn nch
             u[i] = ((cur << sh) | (prev >> (24 - sh))) & 0xffffffu;
     }
     long long z24[NW24];
-#pragma unroll
-    for (int j = 0; j < NW24; ++j) {
-        const int bit = 24 * j;   // compile-time after unrolling
-        const int wi = bit / WB, off = bit % WB;
-        const uint32_t lo = (wi <= NWB) ? u[wi] : 0u;
-        const uint32_t hi = (wi + 1 <= NWB) ? u[wi + 1] : 0u;
-        const uint64_t both = (uint64_t)lo | ((uint64_t)hi << WB);
-        z24[j] = (long long)((both >> off) & 0xffffffu);
-    }
+    recut24<NWB + 1, WB>(u, z24, std::make_integer_sequence<int, NW24>{});
     fold_words<L, NW24, 3>(z24, neg, w, pad, sc0, y);
 }
 
