@@ -278,8 +278,8 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 #pragma unroll
                     for (int v = 0; v < 16; ++v) tot[j][i][jj][v] = 0;
     }
-    // BF16 tiles are moved into tot KT at a time, one group per step
-    // (groups of KT consecutive tiles, NB groups)
+    // BF16 tiles are moved into tot KT at a time, one group per moving step
+    // (groups of KT consecutive tiles, NB groups), and cleared
     constexpr int NT = G * IM * IN;
     constexpr int KT = (NT + 7) / 8, NB = NT / KT;
     static_assert(!BF || NB * KT == NT, "tile groups must cover the tiles");
@@ -327,11 +327,9 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     };
     using J0All = std::integral_constant<int, 0>;
     // MFMAs of one k-step over the NT accumulator tiles (tile T = (j IM + i)
-    // IN + jj) in the order O, O+1, ... mod NT, diagonals j >= J0 only; the
-    // tiles of group ZT >= 0 (BF16 only) start from 0: their sums were
-    // copied out in the step before
-    auto mfma = [&](int buf, auto oc, auto zc, auto jc) {
-        constexpr int O = decltype(oc)::value, ZT = decltype(zc)::value, J0 = decltype(jc)::value;
+    // IN + jj) in the order O, O+1, ... mod NT, diagonals j >= J0 only
+    auto mfma = [&](int buf, auto oc, auto jc) {
+        constexpr int O = decltype(oc)::value, J0 = decltype(jc)::value;
 #pragma unroll
         for (int idx = 0; idx < NT; ++idx) {
             const int T = (O + idx) % NT;
@@ -339,14 +337,14 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
             if (j < J0 || i >= IMV) continue;
             if constexpr (BF)
                 acc[j][i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                    __builtin_bit_cast(bnd_bf8, a[buf][i]), __builtin_bit_cast(bnd_bf8, b[buf][j][jj]),
-                    (ZT >= 0 && T / KT == ZT) ? (bnd_f16){} : acc[j][i][jj], 0, 0, 0);
+                    __builtin_bit_cast(bnd_bf8, a[buf][i]), __builtin_bit_cast(bnd_bf8, b[buf][j][jj]), acc[j][i][jj],
+                    0, 0, 0);
             else
                 acc[j][i][jj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[buf][i], b[buf][j][jj], acc[j][i][jj], 0, 0, 0);
         }
     };
     using C0 = std::integral_constant<int, 0>;
-    using NoZero = std::integral_constant<int, -1>;
+    using None = std::integral_constant<int, -1>;
 
     // The barrier of step s sits inside its last k-step: wait for this
     // wave's loads of step s+1 (and for its own fragment reads), barrier,
@@ -363,12 +361,12 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     set_step();
     fetch(0, 0, J0All{});
     // BF16 moves, a group of tiles at a time: a step with NX >= 0 runs group
-    // NX first in its last k-step and adds its sums into tot under the other
-    // tiles' MFMAs; a step with FT >= 0 restarts group FT (added into tot at
-    // the end of the step before) from 0 in its first k-step, where those
-    // tiles go last.  J0: the step's first diagonal with a slice pair (the
-    // MFMAs of diagonals j < J0 would only add zero products and are left
-    // out; J0 = G: a padding step, copies and barrier only)
+    // NX first in its last k-step, adds its sums into tot and clears them
+    // under the other tiles' MFMAs; in its first k-step the tiles of group
+    // FT >= 0 (cleared by the step before) go last.  J0: the step's first
+    // diagonal with a slice pair (the MFMAs of diagonals j < J0 would only
+    // add zero products and are left out; J0 = G: a padding step, copies
+    // and barrier only)
     auto step = [&](auto fc, auto nc, auto jc) {
         constexpr int FT = decltype(fc)::value, NX = decltype(nc)::value, J0 = decltype(jc)::value;
         // the scheduler places one fragment read (or one global->LDS copy)
@@ -379,9 +377,9 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         for (int ks = 0; ks < 3; ++ks) {
             fetch((ks & 1) ^ 1, ks + 1, jc);
             if (ks == 0)
-                mfma(0, std::integral_constant<int, ((FT + 1) * KT) % NT>{}, fc, jc);
+                mfma(0, std::integral_constant<int, ((FT + 1) * KT) % NT>{}, jc);
             else
-                mfma(ks & 1, C0{}, NoZero{}, jc);
+                mfma(ks & 1, C0{}, jc);
 #pragma unroll
             for (int r = 0; r < (NR < NM ? NR : NM); ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
@@ -400,7 +398,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         fetch(0, 0, J0All{});
         issue_a();
         issue_b();
-        mfma(1, std::integral_constant<int, (NX > 0 ? NX * KT : 0)>{}, NoZero{}, jc);
+        mfma(1, std::integral_constant<int, (NX > 0 ? NX * KT : 0)>{}, jc);
         constexpr int NR1 = NR0 < NM ? NR0 : NM;
         constexpr int NC = (NM - NR1) < 8 ? (NM - NR1) : 8;
         if constexpr (BF && NX >= 0) {
@@ -410,6 +408,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
                 const int j = T / (IM * IN), i = (T / IN) % IM, jj = T % IN;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) tot[j][i][jj][v] += (int)acc[j][i][jj][v];
+                acc[j][i][jj] = AccT{};
             }
 #pragma unroll
             for (int r = 0; r < NR1; ++r) {
@@ -417,8 +416,8 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // LDS read
             }
             // after the group's own MFMAs (KT <= NR): MFMA, copy and a share
-            // of the 48 KT moves (read, convert, add)
-            constexpr int NV = (48 * KT + 7) / 8;
+            // of the 64 KT moves (read, convert, add; clear)
+            constexpr int NV = (64 * KT + 7) / 8;
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
@@ -439,27 +438,40 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         }
         __builtin_amdgcn_sched_barrier(0);
     };
-    bool drop = false;   // BF16: group NB-1's acc is already in tot
     if constexpr (BF) {
         // FP32 sums stay exact below 2^24, i.e. for 16 steps (1024 products
-        // of at most 2^14).  In blocks of NB steps, step r restarts group
-        // (r - 1) mod NB and adds group r into tot, so each group is added
-        // and restarted once per NB steps (the first restart of group NB-1
-        // finds 0).  After the last block group NB-1 is in tot but not
-        // restarted: the first of the other nsteps % NB steps restarts it,
-        // and with none left the epilogue leaves its acc out (drop).  Group
-        // r < NB-1 then holds NB-1-r + nsteps % NB <= 2 NB - 2 steps.
-        static_assert(2 * NB - 2 <= 16, "tiles must be restarted within 16 steps");
-        int s = 0;
-        for (; s + NB <= nsteps; s += NB)
-            bnd_unroll(std::make_integer_sequence<int, NB>{}, [&](auto rc) {
-                constexpr int R = decltype(rc)::value;
-                step(std::integral_constant<int, (R + NB - 1) % NB>{}, rc, J0All{});
+        // of at most 2^14).  Moving step R (0 <= R < NB) adds group R into
+        // tot and clears it; the group gathers the MFMAs of the steps up to
+        // its next move.  The steps run in segments of nm full steps -- the
+        // moving steps R = 0, 1, .., NB-1 as often as they fit, the other
+        // nm % NB without a move -- then the steps of diagonals J0 = jlo ..
+        // jhi, then np padding steps.  An item with at least NB full steps
+        // per k-block has a segment per k-block (nm = its full steps) and,
+        // like the INT8 steps, leaves out the MFMAs of diagonals without a
+        // slice pair: a group moved at its step r of a k-block gathers at
+        // most NB-1-r moving, NB-1 other full and G-1 such steps there, and
+        // r+1 steps of the next k-block, 2 NB + G - 2 in all.  Any other item
+        // is one segment of nm = nsteps steps that run every MFMA (zero
+        // blocks where no pair exists).
+        static_assert(2 * NB + G - 2 <= 16, "groups must be moved within 16 steps");
+        const int nfull = max(0, min(nA, g0 + 1 - t0));
+        const bool skip = nfull >= NB;
+        const int nseg = skip ? nkb : 1, nm = skip ? nfull : nsteps;
+        const int jlo = skip ? max(1, t0 + nfull - g0) : G, jhi = skip ? min(G - 1, t1 - 1 - g0) : 0;
+        const int np = skip ? nsk - nA : 0;
+        for (int sg = 0; sg < nseg; ++sg) {
+            int c = 0;
+            for (; c + NB <= nm; c += NB)
+                bnd_unroll(std::make_integer_sequence<int, NB>{}, [&](auto rc) {
+                    constexpr int R = decltype(rc)::value;
+                    step(std::integral_constant<int, (R + NB - 1) % NB>{}, rc, J0All{});
+                });
+            for (; c < nm; ++c) step(None{}, None{}, J0All{});
+            bnd_unroll(std::make_integer_sequence<int, G - 1>{}, [&](auto rc) {
+                constexpr int J = decltype(rc)::value + 1;
+                if (J >= jlo && J <= jhi) step(None{}, None{}, std::integral_constant<int, J>{});
             });
-        drop = s > 0 && s == nsteps;
-        if (s < nsteps) {
-            step(std::integral_constant<int, NB - 1>{}, NoZero{}, J0All{});
-            for (++s; s < nsteps; ++s) step(NoZero{}, NoZero{}, J0All{});
+            for (c = 0; c < np; ++c) step(None{}, None{}, std::integral_constant<int, G>{});
         }
     } else {
         // per k-block: nfull steps with a slice pair on every diagonal, the
@@ -472,12 +484,12 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         // k-block with fewer k-steps made the compiler spill the
         // accumulators: every k-block runs all four k-steps)
         for (int kb = 0; kb < nkb; ++kb) {
-            for (int c = 0; c < nfull; ++c) step(NoZero{}, NoZero{}, J0All{});
+            for (int c = 0; c < nfull; ++c) step(None{}, None{}, J0All{});
             bnd_unroll(std::make_integer_sequence<int, G - 1>{}, [&](auto rc) {
                 constexpr int J = decltype(rc)::value + 1;
-                if (J >= jlo && J <= jhi) step(NoZero{}, NoZero{}, std::integral_constant<int, J>{});
+                if (J >= jlo && J <= jhi) step(None{}, None{}, std::integral_constant<int, J>{});
             });
-            for (int c = 0; c < npad; ++c) step(NoZero{}, NoZero{}, std::integral_constant<int, G>{});
+            for (int c = 0; c < npad; ++c) step(None{}, None{}, std::integral_constant<int, G>{});
         }
     }
     // no global->LDS copy may still be in flight when the workgroup ends
@@ -507,8 +519,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
                     for (int j = 0; j < G; ++j) {
                         int dj;
                         if constexpr (BF)
-                            dj = tot[j][i][jj][v] +
-                                 ((drop && ((j * IM + i) * IN + jj) / KT == NB - 1) ? 0 : (int)acc[j][i][jj][v]);
+                            dj = tot[j][i][jj][v] + (int)acc[j][i][jj][v];
                         else
                             dj = acc[j][i][jj][v];
                         P = P * 256 + dj;

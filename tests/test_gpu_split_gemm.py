@@ -230,9 +230,11 @@ def _digits_127(L, rows, cols):
 def test_band_kernel_largest_partial_sums(mdg, orc, torch_cuda, method):
     # every product of two slices is 127^2 and they all add up, so the BF16
     # kernel's FP32 sums grow as fast as they can between moves into INT32
-    # (at most 14 steps of 64 products: 896 * 127^2 < 2^24; a sum held past
-    # 1040 products, 1041 * 127^2 odd and above 2^24, would round); the
-    # result must match the INT8 path bit for bit and the exact product
+    # (at most 13 steps of 64 products: 832 * 127^2 < 2^24, also where the
+    # steps without a slice pair on every diagonal leave MFMAs out; a sum
+    # held past 1040 products, 1041 * 127^2 odd and above 2^24, would
+    # round); the result must match the INT8 path bit for bit and the exact
+    # product
     L, m, n, k = 4, 64, 96, 4096
     A = _digits_127(L, m, k)
     B = _digits_127(L, k, n)
