@@ -168,6 +168,7 @@ def test_int8_engines_agree_multi_tile(mdg, orc, torch_cuda):
     (8, 129, 257, 128, 30),   # S not a multiple of 4, ragged tiles
     (16, 64, 72, 300, 60),    # ~110 slices: 28 bands
     (4, 24, 40, 40000, 4),    # kp = 40064: chunks of 3 slices, padding steps
+    (4, 40, 36, 20000, 6),    # kp = 20096: chunks of 6 slices (BF16: moves on full steps, padding steps)
     (2, 20, 17, 70000, 2),    # chunks of one slice
 ])
 def test_int8_band_kernel_shapes(mdg, orc, torch_cuda, L, m, n, k, emax):
@@ -213,6 +214,31 @@ def test_band_kernel_every_leftover_band(mdg, orc, torch_cuda, L, m, n, k):
         assert np.array_equal(to_host(C2).view(np.uint64), to_host(C4).view(np.uint64)), S
         assert np.array_equal(to_host(C2).view(np.uint64), to_host(C5).view(np.uint64)), S
     assert {S % 4 for S in seen} == {0, 1, 2, 3} and {S % 3 for S in seen} == {0, 1, 2}, sorted(seen)
+
+
+@pytest.mark.parametrize("method", ["vector"] + METHODS)
+def test_empty_shapes(mdg, orc, torch_cuda, method):
+    # k = 0: C = 0 in every limb; m = 0 or n = 0: nothing to compute and C
+    # left as it was
+    import torch
+    L = 4
+    A, B = to_dev(np.ones(4 * L)), to_dev(np.ones(4 * L))
+
+    def run(m, n, k, C):
+        print(f"empty {method} {m}x{n}x{k}", flush=True)
+        if method == "vector":
+            return mdg.vector_gemm(L, m, n, k, A, B, C=C)
+        return gemm(mdg, L, m, n, k, A, B, method, C=C)[0]
+
+    C = torch.full((3 * 5 * L,), 7.0, dtype=torch.float64, device="cuda")
+    out = run(3, 5, 0, C)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(out).item() == 0
+    C = torch.full((2 * 5 * L,), 7.0, dtype=torch.float64, device="cuda")
+    run(0, 5, 4, C)
+    run(2, 0, 4, C)
+    torch.cuda.synchronize()
+    assert bool(torch.all(C == 7.0))
 
 
 def _digits_127(L, rows, cols):
