@@ -12,7 +12,9 @@
  * library allocates only its own scratch.  `stream` is a hipStream_t (NULL =
  * the null stream); every call is asynchronous on it unless noted.
  * Every entry point returns an mdg_status; mdg_last_error() gives the text
- * of the last failure on the calling thread.
+ * of the last failure on the calling thread.  Host threads may call the
+ * library at the same time on different streams (scratch is kept per device
+ * and stream); calls from several threads on one stream are not supported.
  *
  * The reference has no error reporting at all (an unsupported p silently
  * returns zeros, C++/TCcuda.cu:269-297); here unsupported limb counts, bad

@@ -241,6 +241,50 @@ def test_empty_shapes(mdg, orc, torch_cuda, method):
     assert bool(torch.all(C == 7.0))
 
 
+@pytest.mark.parametrize("method", ["int8", "bf16", "int8_residue", "f64"])
+def test_concurrent_calls_on_two_streams(mdg, orc, torch_cuda, method):
+    # two host threads, each on a stream of its own (scratch is per stream),
+    # three calls each at once: every result bit-identical to the same call
+    # made alone
+    import threading
+    import torch
+    L = 4
+    shapes = [(150, 130, 300), (96, 200, 257)]
+    data = []
+    for i, (m, n, k) in enumerate(shapes):
+        A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 71 + i, -8, 8))
+        B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 81 + i, -8, 8))
+        want, _ = mdg.gemm(L, m, n, k, A, B, method=method)
+        data.append((m, n, k, A, B, to_host(want)))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [[], []]
+    errs = []
+
+    def work(i):
+        try:
+            m, n, k, A, B, _ = data[i]
+            for _ in range(3):
+                C = mdg.empty(m * n * L)
+                mdg.gemm(L, m, n, k, A, B, method=method, C=C, stream=streams[i])
+                outs[i].append(C)
+        except Exception as e:   # reported by the main thread
+            errs.append(e)
+
+    print(f"concurrent {method}", flush=True)
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errs, errs
+    for i in range(2):
+        assert len(outs[i]) == 3
+        for C in outs[i]:
+            assert np.array_equal(to_host(C).view(np.uint64), data[i][5].view(np.uint64)), (method, i)
+
+
 def _digits_127(L, rows, cols):
     # every entry 1/2 + sum_{t=1}^{6L-1} 127 * 2^(-8t-6), limb l holding the
     # terms t = 6l .. 6l+5 (at most 48 bits, exact): the row scale is 2^0,
