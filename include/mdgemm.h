@@ -237,6 +237,18 @@ int mdg_mixsplit2(const double *x, long long count, double *out, void *stream);
    callers that want the explicit operand; mdg_conv_gemm never builds it. */
 int mdg_bigB2(const double *Bq, int n, int pp, double *out, void *stream);
 
+/* matmulhost (C++/TCcuda.cu:304-317): C[r][c][i] = sum_k sum_{j<=i}
+   A[r][k][j] * B[k][c][i-j] for n x n matrices of p-part entries, in plain
+   doubles with every product and sum rounded, k ascending and j = 0 .. i
+   within it (the reference's host loop order, so the same bits). */
+int mdg_matmulhost(int p, int n, const double *A, const double *B, double *C, void *stream);
+
+/* renormhost (big = 0, interleaved entries A[(i*n + j)*p + k]) and
+   renormhostbig (big = 1, A[i*n*p + k*n + j], the bigA layout),
+   C++/TCcuda.cu:374-398, in place: per entry, k = 0 .. p-2 in turn,
+   hi = x_k + x_{k+1}; x_{k+1} = hi - x_k; x_k = hi. */
+int mdg_renormhost(double *A, int n, int p, int big, void *stream);
+
 /* The reference's FP64 tensor-core kernel `matmul` (C++/TCtest.cu:21-46):
    C (M x N, row-major) += A (M x K, row-major) * B (K x N, column-major), on
    FP64 MFMA.  Accumulates into C like the reference. */

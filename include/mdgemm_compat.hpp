@@ -45,6 +45,14 @@ std::vector<double> matmulqdf(std::vector<double> A, std::vector<double> B, int 
 std::vector<double> matmulodf(std::vector<double> A, std::vector<double> B, int n);
 std::vector<double> matmulhdf(std::vector<double> A, std::vector<double> B, int n);
 
+// C++/TCcuda.h:69 -- limb convolution of p-part entries in plain doubles (the
+// reference's host loop, here on the GPU in the same order: the same bits)
+std::vector<double> matmulhost(std::vector<double> A, std::vector<double> B, int n, int p);
+// C++/TCcuda.h:78,81 -- the reference's host renormalisation of every entry,
+// in place, interleaved (renormhost) or in the bigA layout (renormhostbig)
+void renormhost(std::vector<double> &A, int n, int p);
+void renormhostbig(std::vector<double> &A, int n, int p);
+
 // C++/TCcuda.h:103 -- fold pp parts into p-doubles, kernel time into t_CUDA (ms)
 std::vector<double> pllsqueeze_old(std::vector<double> x, int p, int pp, float &t_CUDA);
 

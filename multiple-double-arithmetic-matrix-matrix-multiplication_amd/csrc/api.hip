@@ -27,6 +27,8 @@ int to_planar(int L, int rows, int cols, const double *x, long long ldx, double 
 int from_planar(int L, int rows, int cols, const double *planar, double *x, long long ldx, hipStream_t s);
 int transpose(int L, int rows, int cols, const double *x, long long ldx, double *y, long long ldy, hipStream_t s);
 int bigB2(const double *Bq, int n, int pp, double *out, hipStream_t s);
+int matmulhost(const double *A, const double *B, int n, int p, double *C, hipStream_t s);
+int renormhost(double *A, int n, int p, int big, hipStream_t s);
 int fill_random(int kind, int L, long long count, unsigned long long seed, int emin, int emax, double *out,
                 hipStream_t s);
 int batched_gemm(int count, const mdg_problem *problems, int method, int target_bits, int spread_bits,
@@ -368,6 +370,25 @@ int mdg_bigB2(const double *Bq, int n, int pp, double *out, void *stream)
     int rc = ensure_device();
     if (rc) return rc;
     return bigB2(Bq, n, pp, out, as_stream(stream));
+}
+
+int mdg_matmulhost(int p, int n, const double *A, const double *B, double *C, void *stream)
+{
+    MDG_REQUIRE(n >= 0 && p >= 1, MDG_ERROR_INVALID_VALUE, "bad n=%d or p=%d", n, p);
+    MDG_REQUIRE(n == 0 || (A && B && C), MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return matmulhost(A, B, n, p, C, as_stream(stream));
+}
+
+int mdg_renormhost(double *A, int n, int p, int big, void *stream)
+{
+    MDG_REQUIRE(n >= 0 && p >= 1, MDG_ERROR_INVALID_VALUE, "bad n=%d or p=%d", n, p);
+    MDG_REQUIRE(big == 0 || big == 1, MDG_ERROR_INVALID_VALUE, "big must be 0 or 1 (got %d)", big);
+    MDG_REQUIRE(n == 0 || A, MDG_ERROR_INVALID_VALUE, "NULL pointer");
+    int rc = ensure_device();
+    if (rc) return rc;
+    return renormhost(A, n, p, big, as_stream(stream));
 }
 
 int mdg_dgemm_acc(int M, int N, int K, const double *A, const double *B, double *C, void *stream)

@@ -171,6 +171,29 @@ std::vector<double> matmulqdf(std::vector<double> A, std::vector<double> B, int 
 std::vector<double> matmulodf(std::vector<double> A, std::vector<double> B, int n) { return host_gemm(A, B, n, 8); }
 std::vector<double> matmulhdf(std::vector<double> A, std::vector<double> B, int n) { return host_gemm(A, B, n, 16); }
 
+std::vector<double> matmulhost(std::vector<double> A, std::vector<double> B, int n, int p)
+{
+    need(n >= 0 && p >= 1, "bad n or p");
+    need((long long)A.size() == (long long)n * n * p && (long long)B.size() == (long long)n * n * p,
+         "A and B must hold n*n*p doubles");
+    DevBuf dA(A), dB(B), dC((size_t)n * n * p);
+    throw_if(mdg_matmulhost(p, n, dA.p, dB.p, dC.p, nullptr), "matmulhost");
+    return dC.host();
+}
+
+static void renorm_in_place(std::vector<double> &A, int n, int p, int big, const char *what)
+{
+    need(n >= 0 && p >= 1, "bad n or p");
+    need((long long)A.size() == (long long)n * n * p, "A must hold n*n*p doubles");
+    DevBuf d(A);
+    throw_if(mdg_renormhost(d.p, n, p, big, nullptr), what);
+    A = d.host();
+}
+
+void renormhost(std::vector<double> &A, int n, int p) { renorm_in_place(A, n, p, 0, "renormhost"); }
+
+void renormhostbig(std::vector<double> &A, int n, int p) { renorm_in_place(A, n, p, 1, "renormhostbig"); }
+
 std::vector<double> pllsqueeze_old(std::vector<double> x, int p, int pp, float &t_CUDA)
 {
     need(pp >= 1 && x.size() % (size_t)pp == 0, "x must hold a multiple of pp doubles");

@@ -184,6 +184,50 @@ __global__ void bigB2_kernel(const double *__restrict__ Bq, int n, int pp, doubl
     out[idx] = (j <= i) ? Bq[((long long)r * n + c) * pp + (i - j)] : 0.0;
 }
 
+// matmulhost (C++/TCcuda.cu:304-317): the limb convolution of two n x n
+// matrices of p-part entries in plain doubles, each product and each sum
+// rounded, in the reference's order -- for entry (r, c) and part i, k
+// ascending and within it j = 0 .. i:
+//   C[r][c][i] = (...(0 + A[r][0][0] B[0][c][i]) + ...) + A[r][n-1][i] B[n-1][c][0]
+// One thread per (r, c, i), i fastest.
+__global__ void matmulhost_kernel(const double *__restrict__ A, const double *__restrict__ B, int n, int p,
+                                  double *__restrict__ C)
+{
+    const long long total = (long long)n * n * p;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int i = (int)(idx % p);
+    const long long rc = idx / p;
+    const long long r = rc / n, c = rc - r * n;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double *a = A + (r * n + k) * p;
+        const double *b = B + ((long long)k * n + c) * p;
+        for (int j = 0; j <= i; ++j) s = s + a[j] * b[i - j];
+    }
+    C[idx] = s;
+}
+
+// renormhost / renormhostbig (C++/TCcuda.cu:374-398), in place: for each
+// entry, k = 0 .. p-2 in turn, hi = x_k + x_{k+1}, x_{k+1} = hi - x_k,
+// x_k = hi (the reference's own rule: the second value is hi - x_k, not the
+// rounding error of the sum).  big = 0: entry (i, j) at A[(i n + j) p + k]
+// (interleaved); big = 1: at A[i n p + k n + j] (the bigA layout).
+__global__ void renormhost_kernel(double *__restrict__ A, int n, int p, int big)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const long long i = idx / n, j = idx - i * n;
+    double *x = big ? A + i * n * p + j : A + idx * p;
+    const long long st = big ? n : 1;
+    for (int k = 0; k < p - 1; ++k) {
+        const double hi = x[k * st] + x[(k + 1) * st];
+        const double lo = hi - x[k * st];
+        x[k * st] = hi;
+        x[(k + 1) * st] = lo;
+    }
+}
+
 // ------------------------------------------------------------- generator --
 __device__ __forceinline__ uint64_t rng_next(uint64_t &s)
 {
@@ -262,6 +306,24 @@ int bigB2(const double *Bq, int n, int pp, double *out, hipStream_t s)
     if (total == 0) return MDG_SUCCESS;
     hipLaunchKernelGGL(bigB2_kernel, dim3(nblocks(total, 256)), dim3(256), 0, s, Bq, n, pp, out);
     MDG_LAUNCH_CHECK("bigB2_kernel");
+    return MDG_SUCCESS;
+}
+
+int matmulhost(const double *A, const double *B, int n, int p, double *C, hipStream_t s)
+{
+    const long long total = (long long)n * n * p;
+    if (total == 0) return MDG_SUCCESS;
+    hipLaunchKernelGGL(matmulhost_kernel, dim3(nblocks(total, 256)), dim3(256), 0, s, A, B, n, p, C);
+    MDG_LAUNCH_CHECK("matmulhost_kernel");
+    return MDG_SUCCESS;
+}
+
+int renormhost(double *A, int n, int p, int big, hipStream_t s)
+{
+    const long long total = (long long)n * n;
+    if (total == 0 || p < 2) return MDG_SUCCESS;
+    hipLaunchKernelGGL(renormhost_kernel, dim3(nblocks(total, 256)), dim3(256), 0, s, A, n, p, big);
+    MDG_LAUNCH_CHECK("renormhost_kernel");
     return MDG_SUCCESS;
 }
 

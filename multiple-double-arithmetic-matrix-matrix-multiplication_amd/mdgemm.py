@@ -280,6 +280,21 @@ def bigB2(Bq, n, pp, stream=None):
     return out
 
 
+def matmulhost(p, n, A, B, stream=None):
+    """The reference's matmulhost (limb convolution in plain doubles, its loop
+    order) on the GPU."""
+    out = empty(n * n * p)
+    _check(lib().mdg_matmulhost(p, n, _ptr(A), _ptr(B), _ptr(out), _stream(stream)))
+    return out
+
+
+def renormhost(A, n, p, big=False, stream=None):
+    """The reference's renormhost (big=False) / renormhostbig (big=True), in
+    place on the device tensor A."""
+    _check(lib().mdg_renormhost(_ptr(A), n, p, 1 if big else 0, _stream(stream)))
+    return A
+
+
 def dgemm_acc(M, N, K, A, B, C, stream=None):
     _check(lib().mdg_dgemm_acc(M, N, K, _ptr(A), _ptr(B), _ptr(C), _stream(stream)))
     return C

@@ -702,6 +702,8 @@ int orc_squeeze(int L, long nent, int pp, const double *x, double *y)
 /*   C1q[r][c][i] = sum_k sum_{j<=i} Aq[r][k][j] * Bq[k][c][i-j]        */
 /* Plain double sums (k outer, j inner); exact whenever every partial   */
 /* sum is representable, the domain the reference's path relies on.    */
+/* The same loop, in the same order, is the reference's host function  */
+/* matmulhost (C++/TCcuda.cu:304-317, square n, p parts).              */
 /* ------------------------------------------------------------------ */
 int orc_conv_gemm(int m, int n, int k, int pp, const double *Aq,
                   const double *Bq, double *C1q)
@@ -717,6 +719,27 @@ int orc_conv_gemm(int m, int n, int k, int pp, const double *Aq,
                              Bq[((long)kk * n + c) * pp + (i - j)];
                 C1q[((long)r * n + c) * pp + i] = s;
             }
+    return 0;
+}
+
+/* renormhost (big = 0: entry (i, j) at A[(i*n + j)*p + k]) and
+   renormhostbig (big = 1: at A[i*n*p + k*n + j]), C++/TCcuda.cu:374-398, in
+   place: per entry, k = 0 .. p-2, hi = x_k + x_{k+1}, x_{k+1} = hi - x_k,
+   x_k = hi. */
+int orc_renormhost(double *A, int n, int p, int big)
+{
+    if (n < 0 || p < 1) return 1;
+    for (long i = 0; i < n; ++i)
+        for (long j = 0; j < n; ++j) {
+            double *x = big ? A + i * n * p + j : A + (i * n + j) * p;
+            const long st = big ? n : 1;
+            for (int k = 0; k < p - 1; ++k) {
+                const double hi = x[k * st] + x[(k + 1) * st];
+                const double lo = hi - x[k * st];
+                x[k * st] = hi;
+                x[(k + 1) * st] = lo;
+            }
+        }
     return 0;
 }
 

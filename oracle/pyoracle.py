@@ -85,6 +85,7 @@ def lib():
         L.orc_conv_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     _dp, _dp, _dp]
         L.orc_bigB2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_renormhost.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_split4pd.argtypes = [ctypes.c_long, ctypes.c_int, _dp, _dp]
         L.orc_split8pd.argtypes = [ctypes.c_long, ctypes.c_int, _dp, _dp]
         L.orc_mixsplit2.argtypes = [ctypes.c_long, _dp, _dp]
@@ -250,6 +251,20 @@ def conv_gemm(m, n, k, pp, Aq, Bq):
     C = np.empty(m * n * pp)
     _chk(lib().orc_conv_gemm(m, n, k, pp, _p(Aq), _p(Bq), _p(C)), "conv_gemm")
     return C
+
+
+def matmulhost(p, n, A, B):
+    """The reference's matmulhost (C++/TCcuda.cu:304-317): orc_conv_gemm with
+    m = n = k, the same loop order."""
+    return conv_gemm(n, n, n, p, A, B)
+
+
+def renormhost(A, n, p, big=False):
+    """The reference's renormhost / renormhostbig (C++/TCcuda.cu:374-398) on
+    a copy of A."""
+    A = _c(A).copy()
+    _chk(lib().orc_renormhost(_p(A), n, p, 1 if big else 0), "renormhost")
+    return A
 
 
 def bigB2(n, pp, Bq):

@@ -70,6 +70,14 @@ int main(int argc, char **argv)
     dump(dir, "part1", part(A, p, 1));
     dump(dir, "bigA", bigA(A, n, p));
     dump(dir, "bigB", bigB(B, n, p));
+    // the reference's other host routines: the limb convolution in plain
+    // doubles, and its two renormalisations (in place)
+    dump(dir, "mh", matmulhost(A, B, n, p));
+    std::vector<double> R1 = A, R2 = bigA(A, n, p);
+    renormhost(R1, n, p);
+    renormhostbig(R2, n, p);
+    dump(dir, "rn", R1);
+    dump(dir, "rnbig", R2);
     // the per-(p, pp) squeeze names on the parts the splitters make
     dump(dir, "sq_p", pllsqueeze(C1q, p, pp));
     dump(dir, "sqnt_p", pllntsqueeze(C1q, p, pp));

@@ -45,6 +45,10 @@ def test_compat_driver(orc, tmp_path, p, n):
         for j in range(i + 1):
             want[i, :, j, :] = B3[:, :, i - j].T
     assert_bits_equal(ld["bigB"], want.ravel(), "bigB")
+    assert_bits_equal(ld["mh"], orc.matmulhost(p, n, A, B), "matmulhost")
+    assert_bits_equal(ld["rn"], orc.renormhost(A, n, p), "renormhost")
+    assert_bits_equal(ld["rnbig"], orc.renormhost(A3.transpose(0, 2, 1).ravel(), n, p, big=True),
+                      "renormhostbig")
     want_sq = orc.squeeze(p, pp, C1q)
     assert_bits_equal(ld["sq_p"], want_sq, "pllsqueeze")
     assert_bits_equal(ld["sqnt_p"], want_sq, "pllntsqueeze")

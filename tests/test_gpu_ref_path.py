@@ -88,3 +88,28 @@ def test_tc_gemm_chain_matches_reference_chain(mdg, orc, torch_cuda, L, kind):
     want = orc.squeeze(L, pp, C1q)
     got = to_host(mdg.tc_gemm(L, n, n, n, to_dev(A), to_dev(B), split_kind=kind))
     assert_bits_equal(got, want, f"tc chain L={L} kind={kind}")
+
+
+@pytest.mark.parametrize("p", [1, 2, 4, 16])
+def test_matmulhost_bit_identical(mdg, orc, torch_cuda, p):
+    # mixed signs and exponents: the sums round, so only the reference's
+    # order (k ascending, j = 0 .. i within it) gives the same bits
+    n = 37
+    rng = np.random.default_rng(40 + p)
+    A = rng.standard_normal(n * n * p) * np.exp2(rng.integers(-30, 30, n * n * p))
+    B = rng.standard_normal(n * n * p) * np.exp2(rng.integers(-30, 30, n * n * p))
+    print(f"matmulhost p={p} n={n}", flush=True)
+    got = to_host(mdg.matmulhost(p, n, to_dev(A), to_dev(B)))
+    assert_bits_equal(got, orc.matmulhost(p, n, A, B), f"matmulhost p={p}")
+
+
+@pytest.mark.parametrize("big", [False, True])
+@pytest.mark.parametrize("p", [2, 4, 8, 16])
+def test_renormhost_bit_identical(mdg, orc, torch_cuda, p, big):
+    n = 45
+    rng = np.random.default_rng(50 + p)
+    A = rng.standard_normal(n * n * p) * np.exp2(rng.integers(-60, 60, n * n * p))
+    print(f"renormhost p={p} big={big}", flush=True)
+    d = to_dev(A)
+    mdg.renormhost(d, n, p, big=big)
+    assert_bits_equal(to_host(d), orc.renormhost(A, n, p, big=big), f"renormhost p={p} big={big}")

@@ -290,3 +290,57 @@ def test_scalar_ops_closed_forms(orc, L):
     # sqr bodies are its mul bodies with b = a), not at L = 2
     if L > 2:
         assert_bits_equal(orc.op(orc.OP_SQR, L, a), orc.op_mul(L, a, a), "sqr == mul(a, a)")
+
+
+def test_matmulhost_small_integers_exact(orc):
+    # every product and partial sum an exact small integer: the convolution
+    # C[r][c][i] = sum_k sum_{j<=i} A[r][k][j] B[k][c][i-j], whatever the order
+    rng = np.random.default_rng(5)
+    for p, n in ((1, 5), (4, 6), (8, 3)):
+        A = rng.integers(-100, 101, size=(n, n, p))
+        B = rng.integers(-100, 101, size=(n, n, p))
+        want = np.zeros((n, n, p), dtype=np.int64)
+        for i in range(p):
+            for j in range(i + 1):
+                want[:, :, i] += A[:, :, j] @ B[:, :, i - j]
+        got = orc.matmulhost(p, n, A.astype(np.float64).ravel(), B.astype(np.float64).ravel())
+        assert np.array_equal(got, want.ravel().astype(np.float64)), (p, n)
+
+
+def test_matmulhost_summation_order(orc):
+    # C++/TCcuda.cu:304-317 adds term by term, k ascending and j = 0 .. i
+    # within each k.  Entry (0, 0), part 1 of a 2 x 2 product of 2-part
+    # entries: terms (k, j) = (0, 0) 2^53, (0, 1) 1, (1, 0) -2^53, (1, 1) 1
+    # give ((2^53 + 1) - 2^53) + 1 = 1 in that order (j outer would give 2)
+    n, p = 2, 2
+    A = np.zeros((n, n, p))
+    B = np.zeros((n, n, p))
+    A[0, 0] = [1.0, 1.0]
+    B[0, 0] = [1.0, 2.0 ** 53]
+    A[0, 1] = [1.0, 1.0]
+    B[1, 0] = [1.0, -(2.0 ** 53)]
+    C = orc.matmulhost(p, n, A.ravel(), B.ravel()).reshape(n, n, p)
+    assert C[0, 0, 1] == 1.0
+    # three terms of one k: (2^53 + 1) + 1 = 2^53 forwards, 2^53 + 2 backwards
+    A1 = np.array([1.0, 1.0, 1.0])
+    B1 = np.array([1.0, 1.0, 2.0 ** 53])
+    assert orc.matmulhost(3, 1, A1, B1)[2] == 2.0 ** 53
+
+
+def test_renormhost_closed_forms(orc):
+    # exact sums: x_k + x_{k+1} and x_{k+1} - ... leave a_k + a_{k+1} in
+    # place k and the last part unchanged; a part below the sum's rounding is
+    # lost (the second value is hi - x_k, not the error of the sum)
+    n, p = 3, 5
+    rng = np.random.default_rng(9)
+    A = rng.integers(-1000, 1001, size=(n, n, p)).astype(np.float64)
+    want = A.copy()
+    want[:, :, :-1] = A[:, :, :-1] + A[:, :, 1:]
+    assert np.array_equal(orc.renormhost(A.ravel(), n, p), want.ravel())
+    x = np.array([1.0, 2.0 ** -60, 0.5])
+    assert np.array_equal(orc.renormhost(x, 1, 3), np.array([1.0, 0.5, 0.5]))
+    # the bigA layout (renormhostbig) holds the same entries with part k of
+    # row i at i n p + k n + j
+    Ar = rng.standard_normal((n, n, p))
+    big = orc.renormhost(Ar.transpose(0, 2, 1).ravel(), n, p, big=True)
+    assert np.array_equal(big, orc.renormhost(Ar.ravel(), n, p).reshape(n, n, p).transpose(0, 2, 1).ravel())
