@@ -32,7 +32,7 @@ extern "C" {
 typedef enum mdg_status {
     MDG_SUCCESS = 0,
     MDG_ERROR_INVALID_VALUE = 1,   /* bad L, shape, leading dimension, pointer */
-    MDG_ERROR_NOT_SUPPORTED = 2,   /* valid request this build cannot serve */
+    MDG_ERROR_NOT_SUPPORTED = 2,   /* valid request the library does not serve */
     MDG_ERROR_HIP = 3,             /* a HIP runtime call failed */
     MDG_ERROR_OUT_OF_MEMORY = 4,   /* scratch allocation failed */
     MDG_ERROR_NON_FINITE = 5,      /* an input limb is Inf or NaN */
