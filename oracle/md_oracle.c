@@ -35,7 +35,14 @@
  *   L3  host GEMM          matmul{dd,qd,od,hd}f  C++/TCcuda.cu:320-372
  *       squeeze            pllsqueeze_{2,4,8,16}_pp_old  C++/TCcuda.cu:876-906
  *       limb convolution   bigB2 + WMMA matmul  C++/TCcuda.cu:79-95,
- *                          C++/TCtest.cu:21-46
+ *                          C++/TCtest.cu:21-46; in plain doubles in the
+ *                          reference's loop order, matmulhost
+ *                          C++/TCcuda.cu:304-317 (terms pinned by exact
+ *                          small-integer products, the order by hand-built
+ *                          rounding cases; tests/test_oracle.py)
+ *       renormhost         renormhost / renormhostbig  C++/TCcuda.cu:374-398
+ *                          (pinned by closed forms: exact sums give
+ *                          x_k + x_{k+1}, a lost tail part gives 0)
  *   L2  splitter           bitform, double_rep, split4, split8, balance,
  *                          split4pd/split8pd/mixsplit2  C++/floatybits.cu:54-342
  *       generator          random_pd  C++/floatybits.cu:73-120 (own PRNG)
