@@ -74,6 +74,8 @@ _SIGS = {
     "mdg_split8pd": ([_vp, _ll, _i, _vp, _vp], _i),
     "mdg_mixsplit2": ([_vp, _ll, _vp, _vp], _i),
     "mdg_bigB2": ([_vp, _i, _i, _vp, _vp], _i),
+    "mdg_matmulhost": ([_i, _i, _vp, _vp, _vp, _vp], _i),
+    "mdg_renormhost": ([_vp, _i, _i, _i, _vp], _i),
     "mdg_dgemm_acc": ([_i, _i, _i, _vp, _vp, _vp, _vp], _i),
     "mdg_conv_gemm": ([_i, _i, _i, _i, _vp, _vp, _vp, _vp], _i),
     "mdg_squeeze": ([_i, _ll, _i, _vp, _vp, _vp], _i),

@@ -50,6 +50,10 @@ def test_bad_shapes_rejected(mdg):
     assert lib.mdg_mixsplit2(None, 3, None, None) == 1
     assert lib.mdg_tc_gemm(4, 4, 4, 4, None, None, None, 12, None) == 1  # mix only for L=2
     assert lib.mdg_fill_random(7, 4, 1, 0, 0, 0, None, None) == 1
+    assert lib.mdg_matmulhost(0, 4, None, None, None, None) == 1          # p < 1
+    assert lib.mdg_matmulhost(4, -1, None, None, None, None) == 1
+    assert lib.mdg_renormhost(None, 4, 4, 2, None) == 1                   # big is 0 or 1
+    assert b"big" in lib.mdg_last_error()
 
 
 def test_null_pointers_rejected(mdg):
