@@ -1063,6 +1063,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     MDG_REQUIRE(zdim <= 65535u, MDG_ERROR_RANGE, "too many slices (%d)", S);
 
     int lib_engine_used = 1;
+    BandTable bf16_tab;   // BF16 band items (filled by the BF16 branch's test)
     if (p.method == MDG_METHOD_INT8_OZAKI) {
         // INT32 planes exact while chunk depth K * 2^14 < 2^31
         const int kmax = (kp <= 131071) ? (131071 / kp) * kp : 130944;
@@ -1130,19 +1131,19 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                                        dim3(256), 0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
             });
         if (rc) return rc;
-    } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071) {
+    } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071 &&
+               build_band_table(S, 3, 131071 / kp, bf16_tab)) {
         // band kernel over blocked BF16 slices: the same exact integer sums
         // as the INT8 path (FP32 partial sums moved into INT32 every 1024
-        // products), as INT64 band planes of 3 diagonals
+        // products), as INT64 band planes of 3 diagonals (192 FP32 sums and
+        // 192 INT32 totals per lane); deeper slices, or more band items than
+        // one launch's table holds, take the single-diagonal kernel below
         const int kmax = (131071 / kp) * kp;
         int nplanes = 0;
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        // bands of 3 diagonals: 192 FP32 sums and 192 INT32 totals per lane
-        BandTable tab;
-        const bool fits = build_band_table(S, 3, kmax / kp, tab);
-        MDG_REQUIRE(fits, MDG_ERROR_RANGE, "too many BF16 band items for %d slices", S);
+        const BandTable &tab = bf16_tab;
         long long *D = (long long *)scratch(s, (size_t)bnd_band_planes(S, 3, kmax / kp) * mn * 8, 4);
         const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);

@@ -331,6 +331,20 @@ def test_int8_deep_k_falls_back_to_tile256(mdg, orc, torch_cuda):
     assert np.array_equal(to_host(C).view(np.uint64), to_host(C2).view(np.uint64)), info2
 
 
+def test_bf16_many_band_items_falls_back(mdg, orc, torch_cuda):
+    # 107 slices at depth 34197 (3 slices per K-chunk): more BF16 band items
+    # than one launch's table holds; the single-diagonal BF16 kernel runs and
+    # gives the INT8 path's bits (this shape used to be refused)
+    L, m, n, k = 16, 24, 20, 34197
+    A = orc.generate(orc.GEN_QD, L, m, k, 100, -46, 46)
+    B = orc.generate(orc.GEN_QD, L, k, n, 200, -46, 46)
+    dA, dB = to_dev(A), to_dev(B)
+    C8, i8 = gemm(mdg, L, m, n, k, dA, dB, "int8", target_bits=824)
+    C5, i5 = gemm(mdg, L, m, n, k, dA, dB, "bf16", target_bits=824)
+    assert i5["slices"] == i8["slices"] > 100, (i5, i8)
+    assert np.array_equal(to_host(C8).view(np.uint64), to_host(C5).view(np.uint64))
+
+
 def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):
     # k > 8192: the FP64 partial sums are moved into INT64 more than once
     L, m, n, k = 4, 40, 36, 20000
@@ -409,3 +423,36 @@ def test_split_gemm_leading_dimensions(mdg, orc, torch_cuda, method, L):
     got = to_host(Cw).reshape(m, ldc, L)
     assert np.array_equal(got[:, :n].view(np.uint64), to_host(C0).reshape(m, n, L).view(np.uint64)), method
     assert np.all(got[:, n:] == 5.0), "padding columns of C were written"
+
+
+def test_random_shapes_engines_agree(mdg, orc, torch_cuda):
+    # a seeded sweep over limb counts, ragged shapes, exponent ranges,
+    # depths (a few deep enough to cut the bands into K-chunks) and
+    # precision targets: the INT8 band kernel, the 256 x 256 kernel and the
+    # BF16 band kernel give the same bits, so do the two residue methods,
+    # and the slice result is within the bound
+    rng = np.random.default_rng(2026)
+    for case in range(36):
+        L = int(rng.choice([2, 4, 8, 16]))
+        m, n = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+        k = int(rng.integers(1, 2500)) if case % 6 else int(rng.integers(15000, 45000))
+        if L == 16 and k > 2500:
+            m, n = min(m, 64), min(n, 64)
+        e = int(rng.integers(0, 8 * L))
+        tb = int(rng.choice([0, 53 * L - 24, 53 * L + 16]))
+        print(f"sweep {case}: L={L} {m}x{n}x{k} e=+-{e} target={tb}", flush=True)
+        A = orc.generate(orc.GEN_QD, L, m, k, 100 + case, -e, e)
+        B = orc.generate(orc.GEN_QD, L, k, n, 200 + case, -e, e)
+        dA, dB = to_dev(A), to_dev(B)
+        C2, i2 = gemm(mdg, L, m, n, k, dA, dB, "int8", target_bits=tb)
+        C4, i4 = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256", target_bits=tb)
+        C5, i5 = gemm(mdg, L, m, n, k, dA, dB, "bf16", target_bits=tb)
+        h2 = to_host(C2)
+        assert np.array_equal(h2.view(np.uint64), to_host(C4).view(np.uint64)), (case, i2, i4)
+        assert np.array_equal(h2.view(np.uint64), to_host(C5).view(np.uint64)), (case, i2, i5)
+        R1, _ = gemm(mdg, L, m, n, k, dA, dB, "int8_residue", target_bits=tb)
+        R2, _ = gemm(mdg, L, m, n, k, dA, dB, "f64_residue", target_bits=tb)
+        assert np.array_equal(to_host(R1).view(np.uint64), to_host(R2).view(np.uint64)), case
+        if tb == 0:
+            err, _ = orc.normwise_error(L, m, n, k, A, B, h2, nsamp=32)
+            assert err <= _bound(L, k), (case, err, i2)
