@@ -342,3 +342,44 @@ def test_batched_loop_plan_survives_scratch_growth_and_release(mdg, orc, torch_c
         torch_cuda.cuda.synchronize()
         for p, w in zip(probs, want):
             assert_bits_equal(to_host(p["C"]), w, f"{method} replay {rep}")
+
+
+@pytest.mark.parametrize("method", ["int8", "int8_residue", "vector"])
+def test_batched_random_mix_persistent_equals_loop(mdg, orc, torch_cuda, method):
+    # seeded batches of every limb count, ragged shapes (1 .. 200), exponent
+    # ranges and padded leading dimensions: the replayed graph against the
+    # loop of single calls bit for bit, and each result within the bound
+    for trial in range(3):
+        rng = np.random.default_rng(700 + trial)
+        probs, host = [], []
+        for i in range(int(rng.integers(8, 20))):
+            L = int(rng.choice([2, 4, 8, 16]))
+            m, n, k = (int(v) for v in rng.integers(1, 201, 3))
+            e = int(rng.integers(0, 7))
+            pad = int(rng.integers(0, 3))
+            lda, ldb, ldc = k + pad, n + pad, n + 2 * pad
+            A = orc.generate(orc.GEN_QD, L, m, lda, 9000 * trial + 2 * i, -e, e)
+            B = orc.generate(orc.GEN_QD, L, k, ldb, 9000 * trial + 2 * i + 1, -e, e)
+            probs.append({"L": L, "m": m, "n": n, "k": k, "A": to_dev(A), "B": to_dev(B), "lda": lda,
+                          "ldb": ldb, "ldc": ldc, "C": to_dev(np.full(m * ldc * L, np.nan))})
+            host.append((L, m, n, k, lda, ldb, ldc, A, B))
+        print(f"batch mix {method} trial {trial}: {len(probs)} problems", flush=True)
+        mdg.gemm_batched(probs, method=method, persistent=0, spread_bits=16)
+        torch_cuda.cuda.synchronize()
+        loop = [to_host(p["C"]).copy() for p in probs]
+        for rep in range(2):
+            for p in probs:
+                p["C"].fill_(np.nan)
+            mdg.gemm_batched(probs, method=method, persistent=1, spread_bits=16)
+            torch_cuda.cuda.synchronize()
+            for i, (p, C) in enumerate(zip(probs, loop)):
+                assert_bits_equal(to_host(p["C"]), C, f"{method} trial {trial} rep {rep} problem {i}")
+        for (L, m, n, k, lda, ldb, ldc, A, B), C in zip(host, loop):
+            A3 = A.reshape(m, lda, L)[:, :k].ravel()
+            B3 = B.reshape(k, ldb, L)[:, :n].ravel()
+            C3 = C.reshape(m, ldc, L)[:, :n].ravel()
+            if method == "vector":
+                assert_bits_equal(C3, orc.gemm(L, m, n, k, A3, B3, mode=orc.TP_FMA), f"vector {L} {m}x{n}x{k}")
+            else:
+                err, _ = orc.normwise_error(L, m, n, k, A3, B3, C3, nsamp=32)
+                assert err <= 4.0 * k * 2.0 ** (-53 * L), (method, L, m, n, k, err)
