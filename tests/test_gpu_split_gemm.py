@@ -5,7 +5,7 @@ arithmetic)."""
 import numpy as np
 import pytest
 
-from conftest import to_dev, to_host
+from conftest import assert_bits_equal, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
@@ -456,3 +456,27 @@ def test_random_shapes_engines_agree(mdg, orc, torch_cuda):
         if tb == 0:
             err, _ = orc.normwise_error(L, m, n, k, A, B, h2, nsamp=32)
             assert err <= _bound(L, k), (case, err, i2)
+
+
+@pytest.mark.parametrize("method", ["vector", "f64", "int8", "bf16", "int8_residue", "f64_residue"])
+def test_power_of_two_scaling_is_exact(mdg, orc, torch_cuda, method):
+    # A 2^a times B 2^b is (A B) 2^(a+b) bit for bit on every path: row and
+    # column scales, slicing, the integer products and the reconstruction
+    # all commute with powers of two, far from 1 in both directions (as long
+    # as no limb leaves the normal range)
+    for L, scales in ((2, ((400, -700), (-500, 480))), (4, ((400, -700), (-500, 480))),
+                      (8, ((300, -500), (-400, 380))), (16, ((150, -180), (-120, 200)))):
+        m, n, k = 37, 29, 45
+        A = orc.generate(orc.GEN_QD, L, m, k, 60 + L, -8, 8)
+        B = orc.generate(orc.GEN_QD, L, k, n, 70 + L, -8, 8)
+
+        def run(X, Y):
+            if method == "vector":
+                return to_host(mdg.vector_gemm(L, m, n, k, to_dev(X), to_dev(Y)))
+            return to_host(gemm(mdg, L, m, n, k, to_dev(X), to_dev(Y), method)[0])
+
+        C = run(A, B)
+        for a, b in scales:
+            print(f"scaling {method} L={L} 2^{a} x 2^{b}", flush=True)
+            Cs = run(np.ldexp(A, a), np.ldexp(B, b))
+            assert_bits_equal(Cs, np.ldexp(C, a + b), f"{method} L={L} 2^{a} 2^{b}")
