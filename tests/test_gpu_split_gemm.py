@@ -480,3 +480,21 @@ def test_power_of_two_scaling_is_exact(mdg, orc, torch_cuda, method):
             print(f"scaling {method} L={L} 2^{a} x 2^{b}", flush=True)
             Cs = run(np.ldexp(A, a), np.ldexp(B, b))
             assert_bits_equal(Cs, np.ldexp(C, a + b), f"{method} L={L} 2^{a} 2^{b}")
+
+
+@pytest.mark.parametrize("method", ["f64", "int8", "bf16", "int8_residue", "f64_residue"])
+def test_transposed_product_is_the_transpose(mdg, orc, torch_cuda, method):
+    # B^T A^T = (A B)^T bit for bit: A's rows and B's columns get their scales
+    # and slices (or residues) the same way, the exact integer sums commute,
+    # and the reconstruction of entry (r, c) depends on E_r + F_c only
+    for L in (2, 4, 8, 16):
+        m, n, k = 41, 27, 66
+        A = orc.generate(orc.GEN_QD, L, m, k, 80 + L, -20, 20)
+        B = orc.generate(orc.GEN_QD, L, k, n, 90 + L, -20, 20)
+        At = np.ascontiguousarray(A.reshape(m, k, L).transpose(1, 0, 2)).ravel()
+        Bt = np.ascontiguousarray(B.reshape(k, n, L).transpose(1, 0, 2)).ravel()
+        print(f"transpose {method} L={L}", flush=True)
+        C, _ = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
+        Ct, _ = gemm(mdg, L, n, m, k, to_dev(Bt), to_dev(At), method)
+        want = np.ascontiguousarray(to_host(C).reshape(m, n, L).transpose(1, 0, 2)).ravel()
+        assert_bits_equal(to_host(Ct), want, f"{method} L={L}")
