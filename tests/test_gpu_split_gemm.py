@@ -498,3 +498,29 @@ def test_transposed_product_is_the_transpose(mdg, orc, torch_cuda, method):
         Ct, _ = gemm(mdg, L, n, m, k, to_dev(Bt), to_dev(At), method)
         want = np.ascontiguousarray(to_host(C).reshape(m, n, L).transpose(1, 0, 2)).ravel()
         assert_bits_equal(to_host(Ct), want, f"{method} L={L}")
+
+
+@pytest.mark.parametrize("method", ["f64", "int8", "bf16", "int8_residue", "f64_residue", "vector"])
+def test_permutation_invariance(mdg, orc, torch_cuda, method):
+    # permuting A's rows and B's columns permutes C's entries (every path);
+    # permuting the depth (A's columns and B's rows alike) leaves C unchanged
+    # on the split paths, whose depth sums are exact integers (the vector path
+    # rounds as it goes, so its sums depend on the order)
+    rng = np.random.default_rng(31)
+    for L in (2, 4, 16):
+        m, n, k = 33, 21, 150
+        A = orc.generate(orc.GEN_QD, L, m, k, 110 + L, -12, 12).reshape(m, k, L)
+        B = orc.generate(orc.GEN_QD, L, k, n, 120 + L, -12, 12).reshape(k, n, L)
+        pr, pc, pk = rng.permutation(m), rng.permutation(n), rng.permutation(k)
+
+        def run(X, Y):
+            X, Y = to_dev(np.ascontiguousarray(X).ravel()), to_dev(np.ascontiguousarray(Y).ravel())
+            if method == "vector":
+                return to_host(mdg.vector_gemm(L, m, n, k, X, Y)).reshape(m, n, L)
+            return to_host(gemm(mdg, L, m, n, k, X, Y, method)[0]).reshape(m, n, L)
+
+        print(f"permutations {method} L={L}", flush=True)
+        C = run(A, B)
+        assert_bits_equal(run(A[pr], B[:, pc]).ravel(), C[pr][:, pc].ravel(), f"{method} L={L} rows/cols")
+        if method != "vector":
+            assert_bits_equal(run(A[:, pk], B[pk]).ravel(), C.ravel(), f"{method} L={L} depth")
