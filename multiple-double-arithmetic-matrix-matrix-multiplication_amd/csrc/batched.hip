@@ -92,13 +92,9 @@ __global__ void b_init_kernel(const BProb *__restrict__ P, const BEnt *__restric
     if (e.x == 0 && threadIdx.x == 0) *p.flag = 0;
 }
 
-// ceil(E - log2(mean |x0|)), clamped to [0, 256]: the measured exponent
-// spread of one row or column (as the single-problem scans measure it)
-__device__ __forceinline__ int bspread(int e, double sum0, int k)
-{
-    const double sp = (double)e - log2(sum0 / (double)k);
-    return (int)fmin(256.0, fmax(0.0, ceil(sp)));
-}
+// the measured exponent spread of one row or column (exponent_spread, as
+// the single-problem scans measure it)
+__device__ __forceinline__ int bspread(int e, double sum0, int k) { return exponent_spread(e, sum0, k); }
 
 template <int L>
 __global__ __launch_bounds__(256) void b_scan_rows_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)

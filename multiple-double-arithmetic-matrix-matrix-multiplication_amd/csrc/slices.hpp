@@ -22,6 +22,18 @@
 
 namespace mdg {
 
+// The exponent spread of a row or column: ceil(E - log2(mean |x0|)) with
+// mean = sum0 / k, clamped to [0, 256].  With mean in [2^(f-1), 2^f) (f its
+// frexp exponent), log2(mean) lies in [f-1, f), so the ceiling is exactly
+// E - f + 1: taken from the two exponents, with no rounded logarithm, and
+// unchanged when the row or column is scaled by a power of two.
+__device__ __forceinline__ int exponent_spread(int e, double sum0, int k)
+{
+    int f;
+    frexp(sum0 / (double)k, &f);
+    return min(256, max(0, e - f + 1));
+}
+
 // words of the fixed-point register per limb count (W <= 32*NW - 1)
 template <int L>
 struct FixedWords {

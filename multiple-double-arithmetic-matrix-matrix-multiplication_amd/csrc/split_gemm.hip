@@ -103,10 +103,7 @@ __global__ __launch_bounds__(256) void scan_rows_kernel(int m, int k, const doub
     if (lane == 0) {
         E[r] = (emax == INT_MIN) ? ZERO_SCALE : emax;
         if (emax != INT_MIN && sum0 > 0.0) {
-            // spread: E - log2(mean |x0|), rounded up, clamped to [0, 256]
-            const double sp = (double)emax - log2(sum0 / (double)k);
-            const int isp = (int)fmin(256.0, fmax(0.0, ceil(sp)));
-            atomicMax(&stats[0], isp);
+            atomicMax(&stats[0], exponent_spread(emax, sum0, k));
         }
         if (bad) atomicOr(&stats[1], 1);
     }
@@ -169,9 +166,7 @@ __global__ void finish_cols_kernel(int k, int n, int *__restrict__ E,
     }
     const double s = colsum[c];
     if (s > 0.0) {
-        const double sp = (double)e - log2(s / (double)k);
-        const int isp = (int)fmin(256.0, fmax(0.0, ceil(sp)));
-        atomicMax(&stats[2], isp);
+        atomicMax(&stats[2], exponent_spread(e, s, k));
     }
 }
 

@@ -175,6 +175,40 @@ def test_metric_config_qd_4096(mdg, orc, torch_cuda):
         assert_bits_equal(Cv[r].ravel(), want, f"vector row {r}")
 
 
+def test_metric_config_qd_4096_invariants(mdg, orc, torch_cuda):
+    # size-independent properties of the INT8 slice path at the benchmark's
+    # full size, compared on the device: permuting the depth (A's columns and
+    # B's rows alike) leaves C unchanged bit for bit (exact integer sums), and
+    # permuting A's rows and B's columns permutes C; scaling A by 2^37 and B
+    # by 2^-90 scales C exactly
+    torch = torch_cuda
+    L, n = 4, 4096
+    A = mdg.fill_random(mdg.GEN_QD, L, n, n, 1, -8, 8)
+    B = mdg.fill_random(mdg.GEN_QD, L, n, n, 2, -8, 8)
+    C, _ = mdg.gemm(L, n, n, n, A, B, method="int8")
+    g = torch.Generator(device="cpu").manual_seed(5)
+    pk = torch.randperm(n, generator=g).cuda()
+    pr = torch.randperm(n, generator=g).cuda()
+    pc = torch.randperm(n, generator=g).cuda()
+    A3, B3 = A.view(n, n, L), B.view(n, n, L)
+    Ak = A3.index_select(1, pk).contiguous().view(-1)
+    Bk = B3.index_select(0, pk).contiguous().view(-1)
+    Ck, _ = mdg.gemm(L, n, n, n, Ak, Bk, method="int8")
+    print("metric qd n=4096: depth permutation", flush=True)
+    assert torch.equal(Ck.view(torch.int64), C.view(torch.int64))
+    del Ak, Bk, Ck
+    Ar = A3.index_select(0, pr).contiguous().view(-1)
+    Bc = B3.index_select(1, pc).contiguous().view(-1)
+    Crc, _ = mdg.gemm(L, n, n, n, Ar, Bc, method="int8")
+    want = C.view(n, n, L).index_select(0, pr).index_select(1, pc).contiguous().view(-1)
+    print("metric qd n=4096: row and column permutations", flush=True)
+    assert torch.equal(Crc.view(torch.int64), want.view(torch.int64))
+    del Ar, Bc, Crc, want
+    Cs, _ = mdg.gemm(L, n, n, n, A * 2.0 ** 37, B * 2.0 ** -90, method="int8")
+    print("metric qd n=4096: power-of-two scaling", flush=True)
+    assert torch.equal(Cs.view(torch.int64), (C * 2.0 ** -53).view(torch.int64))
+
+
 @pytest.fixture(scope="module")
 def hd4096(mdg, torch_cuda):
     L, n = 16, 4096
