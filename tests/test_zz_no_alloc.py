@@ -69,3 +69,24 @@ def test_noalloc_rejects_oversize_and_residue(mdg, orc, torch_cuda):
     C1, _ = mdg.gemm(L, m, m, m, B, B, method="int8")
     C2, _ = mdg.gemm_noalloc(L, m, m, m, B, B, method="int8")
     assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64))
+
+
+def test_noalloc_random_shapes_match_gemm(mdg, orc, torch_cuda):
+    # a seeded sweep (limb counts, ragged shapes, exponent ranges, one depth
+    # cut into K-chunks): the no-allocation call gives mdg_gemm's bits
+    rng = np.random.default_rng(88)
+    for case in range(12):
+        L = int(rng.choice([2, 4, 8, 16]))
+        m, n = int(rng.integers(1, 160)), int(rng.integers(1, 160))
+        k = int(rng.integers(1, 2000)) if case else 20000
+        if case == 0:   # the deep one: double-double, so its slices fit the workspace
+            L, m, n = 2, min(m, 40), min(n, 40)
+        e = int(rng.integers(0, 4 * L))
+        A = orc.generate(orc.GEN_QD, L, m, k, 300 + case, -e, e)
+        B = orc.generate(orc.GEN_QD, L, k, n, 400 + case, -e, e)
+        dA, dB = to_dev(A), to_dev(B)
+        for method in ("int8", "bf16", "f64", "vector"):
+            print(f"noalloc sweep {case} {method} L={L} {m}x{n}x{k}", flush=True)
+            C1, _ = mdg.gemm(L, m, n, k, dA, dB, method=method)
+            C2, _ = mdg.gemm_noalloc(L, m, n, k, dA, dB, method=method)
+            assert np.array_equal(to_host(C1).view(np.uint64), to_host(C2).view(np.uint64)), (case, method)
