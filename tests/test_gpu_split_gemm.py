@@ -524,3 +524,19 @@ def test_permutation_invariance(mdg, orc, torch_cuda, method):
         assert_bits_equal(run(A[pr], B[:, pc]).ravel(), C[pr][:, pc].ravel(), f"{method} L={L} rows/cols")
         if method != "vector":
             assert_bits_equal(run(A[:, pk], B[pk]).ravel(), C.ravel(), f"{method} L={L} depth")
+
+
+@pytest.mark.parametrize("method", ["f64", "int8", "bf16", "int8_residue", "f64_residue"])
+def test_extreme_exponent_ranges_within_the_bound(mdg, orc, torch_cuda, method):
+    # leading limbs spread over 2^-150 .. 2^150 inside every row and column:
+    # the measured spread (E - log2(mean |x0|) <= log2 k + 1) stays small, the
+    # entries far below a row's scale fall under its fixed point, and the
+    # normwise error against |A| |B| still meets the bound at every L
+    for L in (2, 4, 8, 16):
+        m, n, k = 12, 10, 30
+        A = orc.generate(orc.GEN_QD, L, m, k, 5 + L, -150, 150)
+        B = orc.generate(orc.GEN_QD, L, k, n, 6 + L, -150, 150)
+        print(f"extreme range {method} L={L}", flush=True)
+        C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
+        err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C))
+        assert err <= _bound(L, k), (method, L, err, info)
