@@ -110,12 +110,13 @@ __global__ __launch_bounds__(256) void scan_rows_kernel(int m, int k, const doub
 }
 
 // B columns: 64 columns x 4 row groups per block, rows chunked over grid.y;
-// partial results meet in E (atomicMax, pre-filled with INT_MIN) and in
-// colsum (atomicAdd, pre-filled with 0).
+// partial results meet in E (atomicMax, pre-filled with INT_MIN); each
+// block writes its column sums to part[blockIdx.y][c], which the finishing
+// kernel adds in a fixed order (the same bits on every run).
 template <int L>
 __global__ __launch_bounds__(256) void scan_cols_kernel(int k, int n, const double *__restrict__ B,
                                                         long long ldb, int rows_per_block,
-                                                        int *__restrict__ E, double *__restrict__ colsum,
+                                                        int *__restrict__ E, double *__restrict__ part,
                                                         int *__restrict__ stats)
 {
     __shared__ int se[4][64];
@@ -149,13 +150,14 @@ __global__ __launch_bounds__(256) void scan_cols_kernel(int k, int n, const doub
         int e = max(max(se[0][cl], se[1][cl]), max(se[2][cl], se[3][cl]));
         double s = (ss[0][cl] + ss[1][cl]) + (ss[2][cl] + ss[3][cl]);
         if (e != INT_MIN) atomicMax(&E[c], e);
-        if (s != 0.0) atomicAdd(&colsum[c], s);
+        part[(long long)blockIdx.y * n + c] = s;
     }
 }
 
 // finish the column scan: zero columns get ZERO_SCALE, spread into stats[2]
-__global__ void finish_cols_kernel(int k, int n, int *__restrict__ E,
-                                   const double *__restrict__ colsum, int *__restrict__ stats)
+// from the column sums of the nparts row chunks, added in chunk order
+__global__ void finish_cols_kernel(int k, int n, int nparts, int *__restrict__ E,
+                                   const double *__restrict__ part, int *__restrict__ stats)
 {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -164,19 +166,14 @@ __global__ void finish_cols_kernel(int k, int n, int *__restrict__ E,
         E[c] = ZERO_SCALE;
         return;
     }
-    const double s = colsum[c];
+    double s = 0.0;
+    for (int y = 0; y < nparts; ++y) s += part[(long long)y * n + c];
     if (s > 0.0) {
         atomicMax(&stats[2], exponent_spread(e, s, k));
     }
 }
 
 __global__ void fill_int_kernel(int *p, long long nel, int v)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nel) p[i] = v;
-}
-
-__global__ void fill_double_kernel(double *p, long long nel, double v)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nel) p[i] = v;
@@ -980,22 +977,23 @@ template <int L>
 static int scan_scales_l(int m, int n, int k, const double *A, long long lda, const double *B, long long ldb,
                          int fixed_spread, hipStream_t s, int **Eo, int **Fo, int *spread_a, int *spread_b)
 {
-    // scratch: E (m), F (n), stats (8) in slot 0, colsum (n) in slot 1
+    // scratch: E (m), F (n), stats (8) in slot 0; the column sums of the
+    // row chunks (nparts x n) in slot 1
     int *E = (int *)scratch(s, sizeof(int) * (size_t)(m + n + 8), 0);
     MDG_REQUIRE(E, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     int *F = E + m;
     int *stats = F + n;
     g_last_stats = stats;
-    double *colsum = (double *)scratch(s, sizeof(double) * (size_t)n, 1);
-    MDG_REQUIRE(colsum, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
+    const int rpb = 256, nparts = (k + rpb - 1) / rpb;
+    double *part = (double *)scratch(s, sizeof(double) * (size_t)nparts * n, 1);
+    MDG_REQUIRE(part, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
     hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, F, (long long)n, INT_MIN);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, stats, 8ll, 0);
-    hipLaunchKernelGGL(fill_double_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, colsum, (long long)n, 0.0);
     hipLaunchKernelGGL((scan_rows_kernel<L>), dim3((m + 3) / 4), dim3(256), 0, s, m, k, A, lda, E, stats);
-    const int rpb = 256;
-    hipLaunchKernelGGL((scan_cols_kernel<L>), dim3((n + 63) / 64, (k + rpb - 1) / rpb), dim3(256), 0, s, k, n, B,
-                       ldb, rpb, F, colsum, stats);
-    hipLaunchKernelGGL(finish_cols_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, k, n, F, colsum, stats);
+    hipLaunchKernelGGL((scan_cols_kernel<L>), dim3((n + 63) / 64, nparts), dim3(256), 0, s, k, n, B, ldb, rpb, F,
+                       part, stats);
+    hipLaunchKernelGGL(finish_cols_kernel, dim3(blocks_for<int>(n, 256)), dim3(256), 0, s, k, n, nparts, F, part,
+                       stats);
     MDG_LAUNCH_CHECK("scan kernels");
     if (fixed_spread < 0) {
         int hstats[4];

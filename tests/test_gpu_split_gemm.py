@@ -540,3 +540,17 @@ def test_extreme_exponent_ranges_within_the_bound(mdg, orc, torch_cuda, method):
         C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), method)
         err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C))
         assert err <= _bound(L, k), (method, L, err, info)
+
+
+@pytest.mark.parametrize("method", ["f64", "int8", "bf16", "int8_residue", "f64_residue"])
+def test_repeated_calls_are_bit_identical(mdg, orc, torch_cuda, method):
+    # no step of a call depends on the order in which workgroups finish (the
+    # column scan adds its row-chunk sums in a fixed order; the GEMMs' sums
+    # are exact): three calls give the same bits and the same slice count
+    L, m, n, k = 4, 70, 300, 3000
+    A = to_dev(orc.generate(orc.GEN_QD, L, m, k, 501, -20, 20))
+    B = to_dev(orc.generate(orc.GEN_QD, L, k, n, 502, -20, 20))
+    runs = [gemm(mdg, L, m, n, k, A, B, method) for _ in range(3)]
+    for C, info in runs[1:]:
+        assert info["slices"] == runs[0][1]["slices"] and info["spread_b"] == runs[0][1]["spread_b"]
+        assert np.array_equal(to_host(C).view(np.uint64), to_host(runs[0][0]).view(np.uint64))
