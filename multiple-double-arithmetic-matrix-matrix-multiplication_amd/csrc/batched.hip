@@ -220,11 +220,11 @@ __global__ __launch_bounds__(256) void b_slice_kernel(const BProb *__restrict__ 
 // ------------------------------------------------------- INT8 band tiles --
 
 // Entry (prob, tile row, tile column, item): one band item (i8band.hpp) of
-// one 128 x 128 tile of one problem; the item packed as g0 & 0xff | t0 << 8
-// | t1 << 16 (g0 >= -(G-1) as a signed byte), the band width is the
-// launch's G; prob < 0: an empty entry.  An item runs nkb k-blocks times
-// its slice steps, so every workgroup is deep enough to keep its copy
-// pipeline full, however shallow the problem.
+// one 128 x 128 tile of one problem; the item packed as g0 & 0xff | kb0 << 8
+// | kb1 << 19 (g0 >= -(G-1) as a signed byte, k-blocks kb0 .. kb1-1 < 2048),
+// the band width is the launch's G; prob < 0: an empty entry.  An item runs
+// its k-blocks times its slice steps, so every workgroup is deep enough to
+// keep its copy pipeline full, however shallow the problem.
 template <int G>
 __global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict__ P, const BEnt *__restrict__ T)
 {
@@ -232,18 +232,18 @@ __global__ __launch_bounds__(256) void b_band_i8_kernel(const BProb *__restrict_
     const BEnt e = T[blockIdx.x];
     if (e.prob < 0) return;
     const BProb &p = P[e.prob];
-    const int g0 = (int)(signed char)(e.z & 0xff), t0 = (e.z >> 8) & 0xff, t1 = (e.z >> 16) & 0xff;
-    const int nkb = p.kp >> 7, Tc = p.kmax / p.kp;
+    const int g0 = (int)(signed char)(e.z & 0xff), kb0 = (e.z >> 8) & 0x7ff, kb1 = (e.z >> 19) & 0x7ff;
+    const int nkb = p.kp >> 7, t1 = g0 + G;
     const long long tstride = (long long)nkb * p.S * BND_BLK;
     const int8_t *At = p.Aall + e.x * tstride, *Bt = p.Ball + e.y * tstride;
     const long long row0 = (long long)e.x * BND_TM, col0 = (long long)e.y * BND_TM;
     // 32-row MFMA tiles that hold rows below m: the others only multiply the
     // zero padding of the tile and are left out
     switch ((int)min(4ll, (p.m - row0 + 31) >> 5)) {
-    case 1: band_run<G, 2, 1, false, 1>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
-    case 2: band_run<G, 2, 1, false, 2>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
-    case 3: band_run<G, 2, 1, false, 3>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
-    default: band_run<G, 2, 1, false, 4>(lds, At, Bt, p.S, nkb, Tc, g0, t0, t1, row0, col0, p.m, p.n, p.D); break;
+    case 1: band_run<G, 2, 1, false, 1>(lds, At, Bt, p.S, nkb, BND_CAP_I8, g0, 0, t1, kb0, kb1, row0, col0, p.m, p.n, p.D); break;
+    case 2: band_run<G, 2, 1, false, 2>(lds, At, Bt, p.S, nkb, BND_CAP_I8, g0, 0, t1, kb0, kb1, row0, col0, p.m, p.n, p.D); break;
+    case 3: band_run<G, 2, 1, false, 3>(lds, At, Bt, p.S, nkb, BND_CAP_I8, g0, 0, t1, kb0, kb1, row0, col0, p.m, p.n, p.D); break;
+    default: band_run<G, 2, 1, false, 4>(lds, At, Bt, p.S, nkb, BND_CAP_I8, g0, 0, t1, kb0, kb1, row0, col0, p.m, p.n, p.D); break;
     }
 }
 
@@ -265,18 +265,15 @@ __global__ __launch_bounds__(256) void b_reconstruct_kernel(const BProb *__restr
     const int er = p.E[r], fc = p.F[c];
     if (er != BZERO_SCALE && fc != BZERO_SCALE) {
         const long long mn = (long long)p.m * p.n;
-        const int S = p.S, Tc = p.kmax / p.kp;
+        const int S = p.S, nkb = p.kp >> 7;
         // the planes live in global memory (a generic pointer read from the
         // problem table would make every load a flat one)
         const __attribute__((address_space(1))) long long *D =
             (const __attribute__((address_space(1))) long long *)p.D;
         band_fold<L, NWB, 4, CHUNKED>(
             S, er + fc + 4 - 8 * (S + 1),
-            [&](int q, int ch) {
-                const int plane = CHUNKED ? bnd_first_band_plane(S, 4, Tc, q) + ch : q;
-                return (long long)D[(long long)plane * mn + idx];
-            },
-            [&](int q) { return bnd_band_chunks(S, 4, Tc, q); }, y);
+            [&](int plane) { return (long long)D[(long long)plane * mn + idx]; },
+            [&](int q) { return bnd_band_chunks(S, 4, nkb, BND_CAP_I8, q); }, p.nplanes, y);
     }
     double *o = p.C + (r * p.ldc + c) * L;
 #pragma unroll
@@ -432,7 +429,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
         // pair must fit an INT32 plane (the per-problem loop handles deeper k)
         MDG_REQUIRE(b.kp <= 131071 && b.S < 256, MDG_ERROR_NOT_SUPPORTED, "problem %d too deep for the fused stages", i);
         b.kmax = (131071 / b.kp) * b.kp;
-        b.nplanes = bnd_band_planes(b.S, 4, b.kmax / b.kp);
+        b.nplanes = bnd_band_planes(b.S, 4, b.kp / 128, BND_CAP_I8);
         offs[i].aall = take((size_t)b.mp * b.S * b.kp);
         offs[i].ball = take((size_t)b.np * b.S * b.kp);
         offs[i].d = take((size_t)b.nplanes * b.m * b.n * 8);
@@ -544,10 +541,10 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
             for (int i = 0; i < count; ++i) {
                 const BProb &b = pl.probs[i];
                 if (b.S <= 0 || b.m == 0 || b.n == 0 || b.k == 0) continue;
-                band_items(b.S, 4, b.kmax / b.kp, its[i]);
+                band_items(b.S, 4, b.kp / 128, BND_CAP_I8, its[i]);
                 std::stable_sort(its[i].begin(), its[i].end(),
                                  [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
-                const long long per_item = (long long)(b.mp / 128) * (b.np / 128) * (b.kp / 128);
+                const long long per_item = (long long)(b.mp / 128) * (b.np / 128);   // item work counts its k-blocks
                 long long w = 0;
                 for (const BandWork &bw : its[i]) w += bw.work * per_item;
                 pws.push_back(PW{i, w});
@@ -560,7 +557,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 const int i = pws[r].prob;
                 const BProb &b = pl.probs[i];
                 for (const BandWork &bw : its[i]) {
-                    const int code = (bw.it.g0 & 0xff) | (bw.it.t0 << 8) | (bw.it.t1 << 16);
+                    const int code = (bw.it.g0 & 0xff) | (bw.it.kb0 << 8) | (bw.it.kb1 << 19);
                     for (int tx = 0; tx < b.mp / 128; ++tx)
                         for (int ty = 0; ty < b.np / 128; ++ty) lists[x].push_back(BEnt{i, tx, ty, code});
                 }
@@ -586,7 +583,7 @@ static int build_plan(Plan &pl, int count, const mdg_problem *pr, int method, in
                 for (const BProb &b : pl.probs)
                     if (b.L == L && b.S) {
                         pl.stages.back().nw = std::max(pl.stages.back().nw, b.S);   // the largest slice count
-                        pl.stages.back().chunked = pl.stages.back().chunked || b.kmax / b.kp < b.S;
+                        pl.stages.back().chunked = pl.stages.back().chunked || b.nplanes > (b.S + 3) / 4;
                     }
         }
     } else if (!res) {

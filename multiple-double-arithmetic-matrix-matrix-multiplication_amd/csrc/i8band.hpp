@@ -19,12 +19,14 @@
 // and the 16-byte MFMA fragment reads of 16 consecutive rows then hit 16
 // distinct 4-bank groups (and 8 rows, 8 groups): no bank conflicts.
 //
-// Exactness: |d_t d_u| <= 2^14, so an INT32 sum over depth K is exact while
-// K < 2^17; a band's t-range is cut into chunks of Tc slices with
-// Tc * kp <= 131071, and each (band, chunk) writes one INT64 "band plane":
-// its G diagonal sums combined exactly, P = sum_j D_{g0+j} 2^(8 (G-1-j))
-// (|P| < 2^56), half the bytes of one INT32 plane per diagonal
-// (recon.hpp band_fold takes them apart again).
+// Exactness: |d_t d_u| <= 2^14, so an INT32 sum of at most 131071 products
+// is exact.  A diagonal of band q sums at most n_t = g0_q + G slice pairs
+// per element of depth, so the band's k-blocks are cut into K-chunks of
+// floor(cap / n_t) k-blocks (cap = floor(131071 / elements per k-block)),
+// and each (band, chunk) writes one INT64 "band plane": its G diagonal sums
+// combined exactly, P = sum_j D_{g0+j} 2^(8 (G-1-j)) (|P| < 2^56), half
+// the bytes of one INT32 plane per diagonal (recon.hpp band_fold takes them
+// apart again and adds the chunks).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -54,13 +56,17 @@ __host__ __device__ __forceinline__ long long bnd_block(int tile, int kb, int t,
     return (((long long)tile * nkb + kb) * S + t) * (long long)BND_BLK;
 }
 
-// One work item of the band kernel: diagonals g0 .. g0+gb-1, slices t of A
-// in [t0, t1), writing chunk c = t0 / Tc of each diagonal's planes.  g0 < 0
-// for the band of leftover diagonals (band_items): diagonals below 0 are
-// empty.
+// One work item of the band kernel: diagonals g0 .. g0+gb-1 over k-blocks
+// [kb0, kb1) (one K-chunk), every slice t of A with a pair in the band,
+// t = 0 .. g0+gb-1.  g0 < 0 for the band of leftover diagonals
+// (band_items): diagonals below 0 are empty.
 struct BandItem {
-    short g0, gb, t0, t1;
+    short g0, gb, kb0, kb1;
 };
+
+// products per INT32 accumulator of one K-chunk, over elements per k-block
+constexpr int BND_CAP_I8 = 131071 / BND_BK;          // INT8: 128 per k-block
+constexpr int BND_CAP_BF16 = 131071 / (BND_BK / 2);  // BF16: 64 per k-block
 
 constexpr int BND_MAX_ITEMS = 240;
 
@@ -126,24 +132,50 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
 
 // Band planes.  Band q = 0 .. nb-1, nb = ceil(S / G), holds diagonals g0_q
 // .. g0_q + G - 1 with g0_q = S - G (nb - q) (band 0 starts below diagonal
-// 0 when G does not divide S, band_items) and has ceil((g0_q + G) / Tc)
-// K-chunks; plane (q, c) is bnd_first_band_plane(q) + c.
-__host__ __device__ __forceinline__ int bnd_band_chunks(int S, int G, int Tc, int q)
+// 0 when G does not divide S, band_items); its n_t = g0_q + G slices of A
+// give K-chunks of floor(cap / n_t) k-blocks, ceil(nkb / that) of them;
+// plane (q, c) is bnd_first_band_plane(q) + c.
+__host__ __device__ __forceinline__ int bnd_band_kblocks(int S, int G, int cap, int q)
 {
     const int nb = (S + G - 1) / G;
-    return (S - G * (nb - q) + G + Tc - 1) / Tc;
+    const int nt = S - G * (nb - q) + G;
+    return max(1, cap / nt);
 }
 
-__host__ __device__ __forceinline__ int bnd_first_band_plane(int S, int G, int Tc, int q)
+__host__ __device__ __forceinline__ int bnd_band_chunks(int S, int G, int nkb, int cap, int q)
+{
+    const int kc = bnd_band_kblocks(S, G, cap, q);
+    return (nkb + kc - 1) / kc;
+}
+
+__host__ __device__ __forceinline__ int bnd_first_band_plane(int S, int G, int nkb, int cap, int q)
 {
     int p = 0;
-    for (int h = 0; h < q; ++h) p += bnd_band_chunks(S, G, Tc, h);
+    for (int h = 0; h < q; ++h) p += bnd_band_chunks(S, G, nkb, cap, h);
     return p;
 }
 
-__host__ __device__ __forceinline__ int bnd_band_planes(int S, int G, int Tc)
+__host__ __device__ __forceinline__ int bnd_band_planes(int S, int G, int nkb, int cap)
 {
-    return bnd_first_band_plane(S, G, Tc, (S + G - 1) / G);
+    return bnd_first_band_plane(S, G, nkb, cap, (S + G - 1) / G);
+}
+
+// the chunk count of every band, for the reconstruction (a kernel argument:
+// no division per band on the device); nb <= ceil(255 / 3)
+struct BandPlanes {
+    int nb, total;
+    unsigned short nch[96];
+};
+
+inline BandPlanes bnd_plane_table(int S, int G, int nkb, int cap)
+{
+    BandPlanes t{};
+    t.nb = (S + G - 1) / G;
+    for (int q = 0; q < t.nb && q < 96; ++q) {
+        t.nch[q] = (unsigned short)bnd_band_chunks(S, G, nkb, cap, q);
+        t.total += t.nch[q];
+    }
+    return t;
 }
 
 // Band kernel.  Grid: x = output tiles (tiles_m * tiles_n), y = band items.
@@ -153,7 +185,7 @@ __host__ __device__ __forceinline__ int bnd_band_planes(int S, int G, int Tc)
 // rows x 32 columns) reads 4 A and G B fragments per 4 G MFMAs, against
 // 2 + 2 G for WM = 2: fewer LDS reads, 1-2% faster at quad-double n = 4096.
 //
-// Step s of the workgroup: k-block kb = s / nsk, slice t = t0 + s % nsk.
+// Step s of the workgroup: k-block kb = kb0 + s / nsk, slice t = t0 + s % nsk.
 // The A stream holds A_t[kb] at position s; the B stream holds, per k-block,
 // B_u for u = g0+gb-1-t0 down to max(0, g0-t1+1), and step s needs its
 // positions s .. s+gb-1 (the window).  Position p of the A stream lives in
@@ -172,15 +204,17 @@ constexpr int bnd_lds_bytes()
 
 // One band item of one output tile: Atile / Btile = the tile's blocked
 // slices (block (kb, slice) at (kb * S + slice) * 16 KiB), diagonals
-// g0 .. g0+G-1, slices t of A in [t0, t1); the tile's rows start at row0 of
-// the m x n band planes D (INT64), its columns at col0.  lds:
+// g0 .. g0+G-1, slices t of A in [t0, t1), k-blocks [kb0, kb1) of the nkb
+// (K-chunk kb0 / bnd_band_kblocks of the band, for chunks of cap: see
+// bnd_band_kblocks); the tile's rows start at row0 of the m x n band planes
+// D (INT64), its columns at col0.  lds:
 // bnd_lds_bytes<G, PA>() bytes.  IMV < IM (INT8) keeps only the first IMV 32-row MFMA tiles of
 // every wave: the work on the zero padding of a tile whose rows from
 // row0 + 32 IMV on are beyond m is left out.
 template <int G, int PA, int WM, bool BF = false, int IMV = BND_TM / WM / 32>
 __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__ Atile,
-                                         const int8_t *__restrict__ Btile, int S, int nkb, int Tc, int g0, int t0,
-                                         int t1, long long row0, long long col0, int m, int n,
+                                         const int8_t *__restrict__ Btile, int S, int nkb, int cap, int g0, int t0,
+                                         int t1, int kb0, int kb1, long long row0, long long col0, int m, int n,
                                          long long *__restrict__ D)
 {
     constexpr int RA = PA + 1, RB = PA + G;
@@ -192,7 +226,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     const int nA = t1 - t0;
     const int nsk = (t1 <= g0 + 1) ? nA + gb - 1 : g0 + gb - t0;
     const int uhi = g0 + gb - 1 - t0;
-    const int nsteps = nkb * nsk;
+    const int nsteps = (kb1 - kb0) * nsk;
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -204,11 +238,11 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 
     // issue cursors: A position pa = s + PA, B position pb = s + PA + gb - 1,
     // as (k-block, index within the k-block) and ring slot
-    int a_kb = 0, a_i = 0, a_slot = 0, b_kb = 0, b_i = 0, b_slot = 0;
+    int a_kb = kb0, a_i = 0, a_slot = 0, b_kb = kb0, b_i = 0, b_slot = 0;
     auto issue_a = [&]() {
         int kb = a_kb, i = a_i;
-        if (kb >= nkb) {   // past the end: reload the last block (in bounds, nobody reads it)
-            kb = nkb - 1;
+        if (kb >= kb1) {   // past the end: reload the last block (in bounds, nobody reads it)
+            kb = kb1 - 1;
             i = nsk - 1;
         }
         const int t = t0 + min(i, nA - 1);
@@ -222,8 +256,8 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     };
     auto issue_b = [&]() {
         int kb = b_kb, i = b_i;
-        if (kb >= nkb) {
-            kb = nkb - 1;
+        if (kb >= kb1) {
+            kb = kb1 - 1;
             i = nsk - 1;
         }
         const int8_t *src = Brow + ((long long)kb * S + (uhi - i)) * BND_BLK;
@@ -456,7 +490,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         static_assert(2 * NB + G - 2 <= 16, "groups must be moved within 16 steps");
         const int nfull = max(0, min(nA, g0 + 1 - t0));
         const bool skip = nfull >= NB;
-        const int nseg = skip ? nkb : 1, nm = skip ? nfull : nsteps;
+        const int nseg = skip ? kb1 - kb0 : 1, nm = skip ? nfull : nsteps;
         const int jlo = skip ? max(1, t0 + nfull - g0) : G, jhi = skip ? min(G - 1, t1 - 1 - g0) : 0;
         const int np = skip ? nsk - nA : 0;
         for (int sg = 0; sg < nseg; ++sg) {
@@ -483,7 +517,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
         // (a second instantiation of these steps for a last, partial
         // k-block with fewer k-steps made the compiler spill the
         // accumulators: every k-block runs all four k-steps)
-        for (int kb = 0; kb < nkb; ++kb) {
+        for (int kb = kb0; kb < kb1; ++kb) {
             for (int c = 0; c < nfull; ++c) step(None{}, None{}, J0All{});
             bnd_unroll(std::make_integer_sequence<int, G - 1>{}, [&](auto rc) {
                 constexpr int J = decltype(rc)::value + 1;
@@ -501,7 +535,9 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
     // below 0, holds 0)
     const int nbands = (S + G - 1) / G;
     const int band = (g0 - (S - G * nbands)) / G;
-    long long *plane = D + (long long)(bnd_first_band_plane(S, G, Tc, band) + t0 / Tc) * ((long long)m * n);
+    long long *plane =
+        D + (long long)(bnd_first_band_plane(S, G, nkb, cap, band) + kb0 / bnd_band_kblocks(S, G, cap, band)) *
+                ((long long)m * n);
     // the lane index again from the exec mask, so that no lane-dependent
     // value has to live (in scratch memory) across the main loop
     const int el = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -534,7 +570,7 @@ __device__ __forceinline__ void band_run(int8_t *lds, const int8_t *__restrict__
 // Single problem.  Grid: x = output tiles (tiles_m * tiles_n), y = band items.
 template <int G, int PA, int WM>
 __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
-                                                      int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
+                                                      int S, int nkb, int cap, int tiles_m, int tiles_n, int m, int n,
                                                       BandTable tab, long long *__restrict__ D)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
@@ -542,14 +578,14 @@ __global__ __launch_bounds__(256) void band_i8_kernel(const int8_t *__restrict__
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const long long tstride = (long long)nkb * S * BND_BLK;
-    band_run<G, PA, WM>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, Tc, it.g0, it.t0, it.t1,
-                        (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
+    band_run<G, PA, WM>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, cap, it.g0, 0, it.g0 + G, it.kb0,
+                        it.kb1, (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
 }
 
 // The same over BF16 slices (64 bf16 per block row, nkb = kp / 64).
 template <int G, int PA, int WM>
 __global__ __launch_bounds__(256) void band_bf16_kernel(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
-                                                        int S, int nkb, int Tc, int tiles_m, int tiles_n, int m, int n,
+                                                        int S, int nkb, int cap, int tiles_m, int tiles_n, int m, int n,
                                                         BandTable tab, long long *__restrict__ D)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[bnd_lds_bytes<G, PA>()];
@@ -557,37 +593,36 @@ __global__ __launch_bounds__(256) void band_bf16_kernel(const int8_t *__restrict
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const long long tstride = (long long)nkb * S * BND_BLK;
-    band_run<G, PA, WM, true>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, Tc, it.g0, it.t0, it.t1,
-                              (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
+    band_run<G, PA, WM, true>(lds, Ab + tm * tstride, Bb + tn * tstride, S, nkb, cap, it.g0, 0, it.g0 + G,
+                              it.kb0, it.kb1, (long long)tm * BND_TM, (long long)tn * BND_TM, m, n, D);
 }
 
-// Band items: bands of G diagonals, each band's t-range cut into chunks of
-// Tc slices (work = slice-pair blocks, padding steps included).  When S is
-// not a multiple of G, the S % G leftover diagonals are the cheapest ones,
+// Band items: bands of G diagonals, each band's k-blocks cut into K-chunks
+// (bnd_band_kblocks; work = steps, one per slice of A and k-block).  When S
+// is not a multiple of G, the S % G leftover diagonals are the cheapest ones,
 // 0 .. S%G - 1 (one slice pair for diagonal 0: a band that reuses its
 // operands less should carry little work), and they run as a G-wide band
 // shifted down, g0 = S%G - G < 0: its first G - S%G diagonals are empty.
 // band_run handles that band unchanged -- slice u = g0 + j - t < 0 reads the
 // zero block, the INT8 step types leave those MFMAs out, the epilogue skips
-// diagonals below t0 -- so every item is G wide and one launch of the G
+// diagonals below 0 -- so every item is G wide and one launch of the G
 // kernel covers all of them.
 struct BandWork {
     BandItem it;
     long long work;
 };
 
-inline void band_items(int S, int G, int Tc, std::vector<BandWork> &items)
+inline void band_items(int S, int G, int nkb, int cap, std::vector<BandWork> &items)
 {
     items.clear();
-    const int gpart = S % G;
-    std::vector<int> starts;   // first diagonal of each band
-    if (gpart) starts.push_back(gpart - G);
-    for (int g0 = gpart; g0 < S; g0 += G) starts.push_back(g0);
-    for (const int g0 : starts) {
-        for (int t0 = 0; t0 < g0 + G; t0 += Tc) {
-            const int t1 = std::min(t0 + Tc, g0 + G);
-            const long long steps = (t1 <= g0 + 1) ? (t1 - t0) + G - 1 : g0 + G - t0;
-            items.push_back(BandWork{BandItem{(short)g0, (short)G, (short)t0, (short)t1}, steps * G});
+    const int nb = (S + G - 1) / G;
+    for (int q = 0; q < nb; ++q) {
+        const int g0 = S - G * (nb - q);
+        const int kc = bnd_band_kblocks(S, G, cap, q);
+        for (int kb0 = 0; kb0 < nkb; kb0 += kc) {
+            const int kb1 = std::min(kb0 + kc, nkb);
+            items.push_back(BandWork{BandItem{(short)g0, (short)G, (short)kb0, (short)kb1},
+                                     (long long)(kb1 - kb0) * (g0 + G)});
         }
     }
 }

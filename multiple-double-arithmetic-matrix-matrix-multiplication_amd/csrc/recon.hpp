@@ -200,14 +200,16 @@ __device__ __forceinline__ void recut24(const uint32_t (&u)[NU], long long (&z)[
 // P = sum_j D_{g0_q + j} 2^(w (G-1-j)), so that Z = sum_g D_g 2^(w (S-1-g))
 // = sum_q (sum_c P_qc) 2^(w G (nb-1-q)): band q lands whole on word nb-1-q of
 // a layout of wb = w G-bit words (32 for INT8 bands of 4, 24 for BF16 bands
-// of 3).  P(q, c) returns the plane value, nch(q) the band's chunk count
-// (CHUNKED false: one chunk each, P < 2^(31 + w (G-1)) added whole; true:
-// each P split at wb bits).  The words are carried into two's complement,
+// of 3).  The planes of band q follow those of bands 0 .. q-1; P(p) returns
+// the value of plane p, nch(q) band q's chunk count and nplanes their total
+// (CHUNKED false: one plane per band, plane q, P < 2^(31 + w (G-1)) added
+// whole; true: each P split at wb bits, the planes walked from the last
+// band down with a running base).  The words are carried into two's complement,
 // the magnitude taken, and re-cut into the 24-bit words of Z 2^(8 pad),
 // pad = 3 ceil(S/3) - S, which fold_words takes for w = 8 -- the same
 // pieces, hence the same limbs, as diag_fold over the diagonal planes.
 template <int L, int NWB, int G, bool CHUNKED, typename PlaneFn, typename ChunkFn>
-__device__ __forceinline__ void band_fold(int S, int sc0, PlaneFn P, ChunkFn nch, double (&y)[L])
+__device__ __forceinline__ void band_fold(int S, int sc0, PlaneFn P, ChunkFn nch, int nplanes, double (&y)[L])
 {
     constexpr int w = 8, WB = w * G;
     static_assert(WB == 24 || WB == 32, "band words of 24 or 32 bits");
@@ -216,17 +218,19 @@ __device__ __forceinline__ void band_fold(int S, int sc0, PlaneFn P, ChunkFn nch
     long long z[NWB];
 #pragma unroll
     for (int i = 0; i < NWB; ++i) z[i] = 0;
+    int base = nplanes;   // first plane of band q + 1 (chunked)
 #pragma unroll
     for (int i = 0; i < NWB; ++i) {
         const int q = nb - 1 - i;
         if constexpr (!CHUNKED) {
             // unconditional load (clamped band) and a mask, not a branch
-            z[i] += P(max(q, 0), 0) & -(long long)(q >= 0);
+            z[i] += P(max(q, 0)) & -(long long)(q >= 0);
         } else {
             if (q < 0) break;   // uniform: the same S on every lane
             const int nc = nch(q);
+            base -= nc;
             for (int c = 0; c < nc; ++c) {
-                const long long v = P(q, c);
+                const long long v = P(base + c);
                 z[i] += v & ((1ll << WB) - 1);
                 if (i + 1 < NWB) z[i + 1] += v >> WB;
             }

@@ -746,10 +746,10 @@ __global__ __launch_bounds__(256) void reconstruct_i8_kernel(int m, int n, int S
 }
 
 // Band planes of the band kernels (i8band.hpp, INT8 bands of G = 4 or BF16
-// bands of 3; w = 8): one INT64 plane per (band, K-chunk), unchunked when
-// one chunk holds every slice (Tc >= S)
+// bands of 3; w = 8): one INT64 plane per (band, K-chunk), bp.nch[q] chunks
+// of band q (bnd_plane_table), unchunked when every band is one chunk
 template <int L, int NWB, int G, bool CHUNKED>
-__global__ __launch_bounds__(256) void reconstruct_band_kernel(int m, int n, int S, int Tc,
+__global__ __launch_bounds__(256) void reconstruct_band_kernel(int m, int n, int S, BandPlanes bp,
                                                                const long long *__restrict__ D,
                                                                const int *__restrict__ E, const int *__restrict__ F,
                                                                double *__restrict__ C, long long ldc)
@@ -765,11 +765,8 @@ __global__ __launch_bounds__(256) void reconstruct_band_kernel(int m, int n, int
         const long long mn = (long long)m * n;
         band_fold<L, NWB, G, CHUNKED>(
             S, er + fc + 4 - 8 * (S + 1),
-            [&](int q, int ch) {
-                const int plane = CHUNKED ? bnd_first_band_plane(S, G, Tc, q) + ch : q;
-                return D[(long long)plane * mn + idx];
-            },
-            [&](int q) { return bnd_band_chunks(S, G, Tc, q); }, y);
+            [&](int plane) { return D[(long long)plane * mn + idx]; }, [&](int q) { return (int)bp.nch[q]; },
+            bp.total, y);
     }
     double *o = C + (r * ldc + c) * L;
 #pragma unroll
@@ -777,17 +774,20 @@ __global__ __launch_bounds__(256) void reconstruct_band_kernel(int m, int n, int
 }
 
 template <int L, int G>
-static int launch_band_recon(int m, int n, int S, int Tc, const long long *D, const int *E, const int *F, double *C,
-                             long long ldc, hipStream_t s)
+static int launch_band_recon(int m, int n, int S, int nkb, int cap, const long long *D, const int *E, const int *F,
+                             double *C, long long ldc, hipStream_t s)
 {
     const dim3 grid((unsigned)(((long long)m * n + 255) / 256));
+    const BandPlanes bp = bnd_plane_table(S, G, nkb, cap);
+    MDG_REQUIRE(bp.nb <= 96, MDG_ERROR_RANGE, "%d slices: more bands than the reconstruction's table holds", S);
+    const bool chunked = bp.total > bp.nb;   // a band of several K-chunks
     return with_band_words<L, G>(S, [&](auto nwb) {
         constexpr int NWB = decltype(nwb)::value;
-        if (Tc < S)   // some band has more than one K-chunk
-            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, true>), grid, dim3(256), 0, s, m, n, S, Tc, D, E,
+        if (chunked)
+            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, true>), grid, dim3(256), 0, s, m, n, S, bp, D, E,
                                F, C, ldc);
         else
-            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, false>), grid, dim3(256), 0, s, m, n, S, Tc, D, E,
+            hipLaunchKernelGGL((reconstruct_band_kernel<L, NWB, G, false>), grid, dim3(256), 0, s, m, n, S, bp, D, E,
                                F, C, ldc);
     });
 }
@@ -825,10 +825,10 @@ enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
 
 // Band table of one problem (band_items: every item G wide, the leftover
 // diagonals as a shifted band), heaviest items first.
-static bool build_band_table(int S, int G, int Tc, BandTable &tab)
+static bool build_band_table(int S, int G, int nkb, int cap, BandTable &tab)
 {
     std::vector<BandWork> v;
-    band_items(S, G, Tc, v);
+    band_items(S, G, nkb, cap, v);
     if ((int)v.size() > BND_MAX_ITEMS) return false;
     std::stable_sort(v.begin(), v.end(), [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
     tab.count = (int)v.size();
@@ -838,24 +838,24 @@ static bool build_band_table(int S, int G, int Tc, BandTable &tab)
 
 // one launch of the band kernel over the problem's table: bands of 4
 // diagonals on INT8 slices
-static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
+static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int cap,
                        int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n, m,
+    hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m, tiles_n, m,
                        n, tab, D);
     MDG_LAUNCH_CHECK("band_i8_kernel");
     return MDG_SUCCESS;
 }
 
 // the same over blocked BF16 slices (nkb = kp / 64), bands of 3 diagonals
-static int launch_band_bf16(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int Tc,
+static int launch_band_bf16(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int cap,
                             int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
 {
     if (tab.count == 0) return MDG_SUCCESS;
     const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, Tc, tiles_m, tiles_n,
+    hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m, tiles_n,
                        m, n, tab, D);
     MDG_LAUNCH_CHECK("band_bf16_kernel");
     return MDG_SUCCESS;
@@ -1068,10 +1068,11 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         const int forced = noalloc_active() ? 0 : int8_backend();
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
         BandTable tab;
-        if (engine == I8E_BAND && (kp > 131071 || !build_band_table(S, 4, kmax / kp, tab))) engine = I8E_TILE256;
+        if (engine == I8E_BAND && (kp > 131071 || !build_band_table(S, 4, kp / 128, BND_CAP_I8, tab)))
+            engine = I8E_TILE256;
         // the band kernel writes one INT64 plane per (band, chunk), the
         // other engines one INT32 plane per (diagonal, chunk)
-        const int nbp = (engine == I8E_BAND) ? bnd_band_planes(S, 4, kmax / kp) : 0;
+        const int nbp = (engine == I8E_BAND) ? bnd_band_planes(S, 4, kp / 128, BND_CAP_I8) : 0;
         void *D = scratch(s, (engine == I8E_BAND) ? (size_t)nbp * mn * 8 : (size_t)nplanes * mn * 4, 4);
         MDG_REQUIRE(D, MDG_ERROR_OUT_OF_MEMORY, "scratch allocation failed");
         if (engine == I8E_BAND) {
@@ -1084,7 +1085,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             launch_slice_blk<L, true>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-            rc = launch_band(tab, Aall, Ball, S, kp / 128, kmax / kp, tiles_mb, tiles_nb, m, n, (long long *)D, s);
+            rc = launch_band(tab, Aall, Ball, S, kp / 128, BND_CAP_I8, tiles_mb, tiles_nb, m, n, (long long *)D, s);
             if (rc) return rc;
         } else {
             // row-major K-contiguous slices: A_all[r][t kp + kk], Bt_all[c][(S-1-u) kp + kk]
@@ -1112,7 +1113,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         p.backend = engine;
         tm.mark(3);
         if (engine == I8E_BAND)
-            rc = launch_band_recon<L, 4>(m, n, S, kmax / kp, (const long long *)D, E, F, C, ldc, s);
+            rc = launch_band_recon<L, 4>(m, n, S, kp / 128, BND_CAP_I8, (const long long *)D, E, F, C, ldc, s);
         else
             rc = with_diag_words<L>(S, w, [&](auto nw) {
                 constexpr int NW = decltype(nw)::value;
@@ -1125,7 +1126,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             });
         if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071 &&
-               build_band_table(S, 3, 131071 / kp, bf16_tab)) {
+               build_band_table(S, 3, kp / 64, BND_CAP_BF16, bf16_tab)) {
         // band kernel over blocked BF16 slices: the same exact integer sums
         // as the INT8 path (FP32 partial sums moved into INT32 every 1024
         // products), as INT64 band planes of 3 diagonals (192 FP32 sums and
@@ -1137,7 +1138,7 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
         const BandTable &tab = bf16_tab;
-        long long *D = (long long *)scratch(s, (size_t)bnd_band_planes(S, 3, kmax / kp) * mn * 8, 4);
+        long long *D = (long long *)scratch(s, (size_t)bnd_band_planes(S, 3, kp / 64, BND_CAP_BF16) * mn * 8, 4);
         const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
         int8_t *Ball = (int8_t *)scratch(s, (size_t)npb * S * kp * 2, 3);
@@ -1146,11 +1147,11 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         launch_slice_blk<L, true, __bf16>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
         tm.mark(2);
         const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-        rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, kmax / kp, tiles_mb, tiles_nb, m, n, D, s);
+        rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, BND_CAP_BF16, tiles_mb, tiles_nb, m, n, D, s);
         if (rc) return rc;
         p.backend = I8E_BAND;
         tm.mark(3);
-        rc = launch_band_recon<L, 3>(m, n, S, kmax / kp, D, E, F, C, ldc, s);
+        rc = launch_band_recon<L, 3>(m, n, S, kp / 64, BND_CAP_BF16, D, E, F, C, ldc, s);
         if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
         // depth beyond one INT32 chunk slice: INT64 diagonal planes
