@@ -55,7 +55,9 @@ def engine_name(path, backend):
         return ENGINE_NAMES.get(backend, str(backend))
     if path in RESIDUE_ENGINE_OF_PATH:
         return {1: "rocblas", 2: "plain_i8_kernel", 3: "hipblaslt"}.get(backend, str(backend))
-    return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel", "bf16": "band_bf16_kernel",
+    if path == "bf16":
+        return {2: "band_bf16_kernel", 4: "diag_bf16_kernel"}.get(backend, str(backend))
+    return {"f64": "diag_f64_kernel", "f64_residue": "diag_f64_kernel",
             "vector": "vector_gemm_kernel"}.get(path, path)
 
 

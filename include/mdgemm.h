@@ -79,8 +79,10 @@ typedef struct mdg_gemm_info {
        [0] exponent scan, [1] slicing, [2] diagonal GEMMs, [3] reconstruction */
     float stage_ms[4];
     int backend;           /* GEMM engine that ran: 2 this library's MFMA
-                              kernels, 4 the 256 x 256 single-diagonal INT8
-                              kernel, 1 rocBLAS, 3 hipBLASLt (INT8 only) */
+                              kernels (band kernels for slices), 4 a
+                              single-diagonal slice kernel (256 x 256 INT8,
+                              INT64-plane BF16), 1 rocBLAS, 3 hipBLASLt
+                              (INT8 only) */
 } mdg_gemm_info;
 
 /* One problem of a batch (all pointers on the device). */
@@ -109,14 +111,15 @@ int mdg_release_workspace(void);
    when on, mdg_gemm waits for its own last event before returning. */
 int mdg_set_profiling(int on);
 
-/* Engine of the INT8 GEMMs (slice planes and residue products):
-   0 or 2 = this library's MFMA kernels (the default: the band kernel for
-   slice planes, the plain INT8 GEMM for residue products), 1 = rocBLAS,
-   3 = hipBLASLt (the first call of a shape times the algorithms it proposes
-   and keeps the fastest), 4 = the 256 x 256 single-diagonal kernel for
-   slice planes (residue products as 0).  Engines 1, 3 and 4 are there for
-   comparison.  All are exact and give identical results; they differ only
-   in speed.  Process-wide. */
+/* Engine of the INT8 GEMMs (slice planes and residue products) and of the
+   BF16 slice planes: 0 or 2 = this library's MFMA kernels (the default: the
+   band kernels for INT8 and BF16 slice planes, the plain INT8 GEMM for
+   residue products), 1 = rocBLAS, 3 = hipBLASLt (INT8 only; the first call
+   of a shape times the algorithms it proposes and keeps the fastest),
+   4 = the single-diagonal kernels for slice planes (256 x 256 INT8,
+   INT64-plane BF16; residue products as 0).  Engines 1, 3 and 4 are there
+   for comparison.  All are exact and give identical results; they differ
+   only in speed.  Process-wide. */
 int mdg_set_int8_backend(int backend);
 
 /* ----------------------------------------------------------------------

@@ -816,48 +816,52 @@ BlasHandle *blas_handle()
     return b;
 }
 
-// INT8 slice-plane engines (mdg_set_int8_backend): 0 and 2 -- the band
-// kernel (i8band.hpp), or the 256 x 256 single-diagonal kernel where the
-// band kernel cannot run (one slice pair deeper than an INT32 plane allows,
-// k > 131071); 4 -- the single-diagonal kernel; 1 rocBLAS and 3 hipBLASLt --
-// the vendor libraries, kept only as comparison engines.
+// Slice-plane engines (mdg_set_int8_backend): 0 and 2 -- the band kernels
+// (i8band.hpp) for INT8 and BF16 slices, or the single-diagonal kernels
+// where a band kernel cannot run (more than 32767 k-blocks, or more bands
+// than the reconstruction's chunk table holds); 4 -- the single-diagonal
+// kernels (256 x 256 INT8, INT64-plane BF16); 1 rocBLAS and 3 hipBLASLt --
+// the vendor libraries for INT8 slices, kept only as comparison engines.
 enum { I8E_BAND = 2, I8E_LIB = 1, I8E_TILE256 = 4 };
 
-// Band table of one problem (band_items: every item G wide, the leftover
-// diagonals as a shifted band), heaviest items first.
-static bool build_band_table(int S, int G, int nkb, int cap, BandTable &tab)
+// Band items of one problem (band_items: every item G wide, the leftover
+// diagonals as a shifted band), heaviest first; false where the band
+// kernels cannot run it (k-block indices are shorts, the reconstruction's
+// chunk table holds 96 bands).
+static bool build_band_items(int S, int G, int nkb, int cap, std::vector<BandItem> &items)
 {
+    items.clear();
+    if (nkb > 32767 || (S + G - 1) / G > 96) return false;
     std::vector<BandWork> v;
     band_items(S, G, nkb, cap, v);
-    if ((int)v.size() > BND_MAX_ITEMS) return false;
     std::stable_sort(v.begin(), v.end(), [](const BandWork &x, const BandWork &y) { return x.work > y.work; });
-    tab.count = (int)v.size();
-    for (int i = 0; i < tab.count; ++i) tab.item[i] = v[i].it;
+    for (const BandWork &w : v) items.push_back(w.it);
     return true;
 }
 
-// one launch of the band kernel over the problem's table: bands of 4
-// diagonals on INT8 slices
-static int launch_band(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int cap,
-                       int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
+// The band kernel over the problem's items, BND_MAX_ITEMS items per launch
+// (the table is a kernel argument); every item writes its own plane, so the
+// launches are independent.  BF16 selects the kernel over blocked BF16
+// slices (bands of 3 diagonals, nkb = kp / 64), else INT8 (4, kp / 128).
+template <bool BF16>
+static int launch_band(const std::vector<BandItem> &items, const int8_t *Aall, const int8_t *Ball, int S, int nkb,
+                       int cap, int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
 {
-    if (tab.count == 0) return MDG_SUCCESS;
-    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m, tiles_n, m,
-                       n, tab, D);
-    MDG_LAUNCH_CHECK("band_i8_kernel");
-    return MDG_SUCCESS;
-}
-
-// the same over blocked BF16 slices (nkb = kp / 64), bands of 3 diagonals
-static int launch_band_bf16(const BandTable &tab, const int8_t *Aall, const int8_t *Ball, int S, int nkb, int cap,
-                            int tiles_m, int tiles_n, int m, int n, long long *D, hipStream_t s)
-{
-    if (tab.count == 0) return MDG_SUCCESS;
-    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
-    hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m, tiles_n,
-                       m, n, tab, D);
-    MDG_LAUNCH_CHECK("band_bf16_kernel");
+    for (size_t i0 = 0; i0 < items.size(); i0 += BND_MAX_ITEMS) {
+        BandTable tab{};
+        tab.count = (int)std::min(items.size() - i0, (size_t)BND_MAX_ITEMS);
+        for (int i = 0; i < tab.count; ++i) tab.item[i] = items[i0 + i];
+        const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)tab.count);
+        if (BF16) {
+            hipLaunchKernelGGL((band_bf16_kernel<3, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m,
+                               tiles_n, m, n, tab, D);
+            MDG_LAUNCH_CHECK("band_bf16_kernel");
+        } else {
+            hipLaunchKernelGGL((band_i8_kernel<4, 2, 1>), grid, dim3(256), 0, s, Aall, Ball, S, nkb, cap, tiles_m,
+                               tiles_n, m, n, tab, D);
+            MDG_LAUNCH_CHECK("band_i8_kernel");
+        }
+    }
     return MDG_SUCCESS;
 }
 
@@ -1056,7 +1060,9 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
     MDG_REQUIRE(zdim <= 65535u, MDG_ERROR_RANGE, "too many slices (%d)", S);
 
     int lib_engine_used = 1;
-    BandTable bf16_tab;   // BF16 band items (filled by the BF16 branch's test)
+    // inside a no-allocation call only the hand-written kernels run
+    const int forced = noalloc_active() ? 0 : int8_backend();
+    std::vector<BandItem> items;
     if (p.method == MDG_METHOD_INT8_OZAKI) {
         // INT32 planes exact while chunk depth K * 2^14 < 2^31
         const int kmax = (kp <= 131071) ? (131071 / kp) * kp : 130944;
@@ -1064,12 +1070,9 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        // inside a no-allocation call only the hand-written kernels run
-        const int forced = noalloc_active() ? 0 : int8_backend();
         int engine = (forced == 1 || forced == 3) ? I8E_LIB : (forced == 4 ? I8E_TILE256 : I8E_BAND);
-        BandTable tab;
-        if (engine == I8E_BAND && (kp > 131071 || !build_band_table(S, 4, kp / 128, BND_CAP_I8, tab)))
-            engine = I8E_TILE256;
+        // the band kernel cuts any depth into exact INT32 K-chunks
+        if (engine == I8E_BAND && !build_band_items(S, 4, kp / 128, BND_CAP_I8, items)) engine = I8E_TILE256;
         // the band kernel writes one INT64 plane per (band, chunk), the
         // other engines one INT32 plane per (diagonal, chunk)
         const int nbp = (engine == I8E_BAND) ? bnd_band_planes(S, 4, kp / 128, BND_CAP_I8) : 0;
@@ -1085,7 +1088,8 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
             launch_slice_blk<L, true>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
             tm.mark(2);
             const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-            rc = launch_band(tab, Aall, Ball, S, kp / 128, BND_CAP_I8, tiles_mb, tiles_nb, m, n, (long long *)D, s);
+            rc = launch_band<false>(items, Aall, Ball, S, kp / 128, BND_CAP_I8, tiles_mb, tiles_nb, m, n,
+                                    (long long *)D, s);
             if (rc) return rc;
         } else {
             // row-major K-contiguous slices: A_all[r][t kp + kk], Bt_all[c][(S-1-u) kp + kk]
@@ -1125,19 +1129,18 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
                                        dim3(256), 0, s, m, n, S, kp, kmax, w, (const int *)D, E, F, C, ldc);
             });
         if (rc) return rc;
-    } else if (p.method == MDG_METHOD_BF16_OZAKI && kp <= 131071 &&
-               build_band_table(S, 3, kp / 64, BND_CAP_BF16, bf16_tab)) {
+    } else if (p.method == MDG_METHOD_BF16_OZAKI && forced != 4 &&
+               build_band_items(S, 3, kp / 64, BND_CAP_BF16, items)) {
         // band kernel over blocked BF16 slices: the same exact integer sums
         // as the INT8 path (FP32 partial sums moved into INT32 every 1024
         // products), as INT64 band planes of 3 diagonals (192 FP32 sums and
-        // 192 INT32 totals per lane); deeper slices, or more band items than
-        // one launch's table holds, take the single-diagonal kernel below
-        const int kmax = (131071 / kp) * kp;
+        // 192 INT32 totals per lane); beyond 32767 k-blocks the
+        // single-diagonal kernel below runs
+        const int kmax = (kp <= 131071) ? (131071 / kp) * kp : 131008;
         int nplanes = 0;
         for (int g = 0; g < S; ++g) nplanes += i8_chunks(g, kp, kmax);
         p.wide = nplanes > S;
         p.flush_blocks = kmax;
-        const BandTable &tab = bf16_tab;
         long long *D = (long long *)scratch(s, (size_t)bnd_band_planes(S, 3, kp / 64, BND_CAP_BF16) * mn * 8, 4);
         const int mpb = (int)round_up(m, 128), npb = (int)round_up(n, 128);
         int8_t *Aall = (int8_t *)scratch(s, (size_t)mpb * S * kp * 2, 2);
@@ -1147,15 +1150,16 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         launch_slice_blk<L, true, __bf16>(n, k, npb, kp, B, ldb, F, S, w, Ball, s);
         tm.mark(2);
         const int tiles_mb = mpb / 128, tiles_nb = npb / 128;
-        rc = launch_band_bf16(tab, Aall, Ball, S, kp / 64, BND_CAP_BF16, tiles_mb, tiles_nb, m, n, D, s);
+        rc = launch_band<true>(items, Aall, Ball, S, kp / 64, BND_CAP_BF16, tiles_mb, tiles_nb, m, n, D, s);
         if (rc) return rc;
         p.backend = I8E_BAND;
         tm.mark(3);
         rc = launch_band_recon<L, 3>(m, n, S, kp / 64, BND_CAP_BF16, D, E, F, C, ldc, s);
         if (rc) return rc;
     } else if (p.method == MDG_METHOD_BF16_OZAKI) {
-        // depth beyond one INT32 chunk slice: INT64 diagonal planes
+        // single-diagonal kernel: INT64 diagonal planes
         p.wide = true;
+        p.backend = I8E_TILE256;
         p.flush_blocks = 16;
         __bf16 *Aall = (__bf16 *)scratch(s, (size_t)mp * S * kp * 2, 2);
         __bf16 *Ball = (__bf16 *)scratch(s, (size_t)np * S * kp * 2, 3);
@@ -1220,7 +1224,9 @@ static int run_split_gemm(Plan &p, int target_bits, int fixed_spread, const doub
         info->spread_a = p.spread_a;
         info->spread_b = p.spread_b;
         info->flushed = p.wide ? 1 : 0;
-        info->backend = p.method == MDG_METHOD_INT8_OZAKI ? (p.backend == I8E_LIB ? lib_engine_used : p.backend) : 2;
+        info->backend = p.method == MDG_METHOD_INT8_OZAKI   ? (p.backend == I8E_LIB ? lib_engine_used : p.backend)
+                        : p.method == MDG_METHOD_BF16_OZAKI ? p.backend
+                                                            : 2;
         for (int i = 0; i < 4; ++i) info->stage_ms[i] = stage[i];
     }
     return MDG_SUCCESS;

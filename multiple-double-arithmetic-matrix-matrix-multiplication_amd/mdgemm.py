@@ -132,8 +132,9 @@ def set_profiling(on):
 
 
 def set_int8_backend(backend):
-    """INT8 GEMM engine: 0 or 2 this library's MFMA kernels (default), 1
-    rocBLAS, 3 hipBLASLt, 4 the 256 x 256 single-diagonal slice kernel."""
+    """Slice-plane and residue GEMM engine: 0 or 2 this library's MFMA
+    kernels (default: the band kernels), 1 rocBLAS, 3 hipBLASLt (INT8 only),
+    4 the single-diagonal slice kernels (256 x 256 INT8, INT64-plane BF16)."""
     _check(lib().mdg_set_int8_backend(int(backend)))
 
 

@@ -16,7 +16,8 @@
 //                                   "mismatch ..." lines and "cases C"
 //   host_fixed_fold items           the band kernels' work items (band_items)
 //                                   for every slice count, band width, depth
-//                                   and chunk capacity used: each (band,
+//                                   (to 32767 k-blocks) and chunk capacity
+//                                   used: each (band,
 //                                   K-chunk) plane written exactly once and
 //                                   inside the plane array, each band's
 //                                   k-blocks covered once, every chunk's
@@ -169,45 +170,51 @@ static int run_items()
 {
     long long items = 0;
     int bad = 0;
+    auto check = [&](int G, int cap, int S, int nkb) {
+        const int nb = (S + G - 1) / G, np = bnd_band_planes(S, G, nkb, cap);
+        std::vector<int> seen(np, 0);
+        std::vector<std::vector<int>> cover(nb, std::vector<int>(nkb, 0));
+        std::vector<BandWork> v;
+        band_items(S, G, nkb, cap, v);
+        for (const BandWork &w : v) {
+            const BandItem &it = w.it;
+            const int q = (it.g0 - (S - G * nb)) / G;
+            const int plane = bnd_first_band_plane(S, G, nkb, cap, q) + it.kb0 / bnd_band_kblocks(S, G, cap, q);
+            const bool ok = q >= 0 && q < nb && it.gb == G && it.g0 == S - G * (nb - q) && plane >= 0 && plane < np &&
+                            it.kb0 >= 0 && it.kb0 < it.kb1 && it.kb1 <= nkb &&
+                            (long long)(it.g0 + G) * (it.kb1 - it.kb0) <= std::max(cap, it.g0 + G);
+            if (!ok) {
+                if (bad++ < 20)
+                    std::printf("bad item G=%d cap=%d S=%d nkb=%d: g0=%d kb %d..%d plane %d of %d\n", G, cap, S, nkb,
+                                it.g0, it.kb0, it.kb1, plane, np);
+                continue;
+            }
+            ++seen[plane];
+            for (int kb = it.kb0; kb < it.kb1; ++kb) ++cover[q][kb];
+            ++items;
+        }
+        for (int p = 0; p < np; ++p)
+            if (seen[p] != 1 && bad++ < 20)
+                std::printf("bad plane G=%d cap=%d S=%d nkb=%d: plane %d written %d times\n", G, cap, S, nkb, p,
+                            seen[p]);
+        for (int q = 0; q < nb; ++q)
+            for (int kb = 0; kb < nkb; ++kb)
+                if (cover[q][kb] != 1 && bad++ < 20)
+                    std::printf("bad cover G=%d cap=%d S=%d nkb=%d: band %d k-block %d\n", G, cap, S, nkb, q, kb);
+    };
     std::vector<int> nkbs;
     for (int v = 1; v <= 70; ++v) nkbs.push_back(v);
     for (int v : {127, 128, 129, 255, 256, 511, 1023, 2047}) nkbs.push_back(v);
     for (int G : {3, 4})
-        for (int cap : {BND_CAP_I8, BND_CAP_BF16, 40, 7})
+        for (int cap : {BND_CAP_I8, BND_CAP_BF16, 40, 7}) {
             for (int S = 1; S < 256; ++S)
-                for (int nkb : nkbs) {
-                    const int nb = (S + G - 1) / G, np = bnd_band_planes(S, G, nkb, cap);
-                    std::vector<int> seen(np, 0);
-                    std::vector<std::vector<int>> cover(nb, std::vector<int>(nkb, 0));
-                    std::vector<BandWork> v;
-                    band_items(S, G, nkb, cap, v);
-                    for (const BandWork &w : v) {
-                        const BandItem &it = w.it;
-                        const int q = (it.g0 - (S - G * nb)) / G;
-                        const int plane = bnd_first_band_plane(S, G, nkb, cap, q) + it.kb0 / bnd_band_kblocks(S, G, cap, q);
-                        const bool ok = q >= 0 && q < nb && it.gb == G && it.g0 == S - G * (nb - q) && plane >= 0 &&
-                                        plane < np && it.kb0 >= 0 && it.kb0 < it.kb1 && it.kb1 <= nkb &&
-                                        (long long)(it.g0 + G) * (it.kb1 - it.kb0) <= std::max(cap, it.g0 + G);
-                        if (!ok) {
-                            if (bad++ < 20)
-                                std::printf("bad item G=%d cap=%d S=%d nkb=%d: g0=%d kb %d..%d plane %d of %d\n", G,
-                                            cap, S, nkb, it.g0, it.kb0, it.kb1, plane, np);
-                            continue;
-                        }
-                        ++seen[plane];
-                        for (int kb = it.kb0; kb < it.kb1; ++kb) ++cover[q][kb];
-                        ++items;
-                    }
-                    for (int p = 0; p < np; ++p)
-                        if (seen[p] != 1 && bad++ < 20)
-                            std::printf("bad plane G=%d cap=%d S=%d nkb=%d: plane %d written %d times\n", G, cap, S,
-                                        nkb, p, seen[p]);
-                    for (int q = 0; q < nb; ++q)
-                        for (int kb = 0; kb < nkb; ++kb)
-                            if (cover[q][kb] != 1 && bad++ < 20)
-                                std::printf("bad cover G=%d cap=%d S=%d nkb=%d: band %d k-block %d\n", G, cap, S, nkb,
-                                            q, kb);
-                }
+                for (int nkb : nkbs) check(G, cap, S, nkb);
+            // depths of several INT32 planes per slice pair, up to the
+            // 32767 k-blocks a band item can index (the kernels' capacities)
+            if (cap == BND_CAP_I8 || cap == BND_CAP_BF16)
+                for (int S : {1, 2, 15, 30, 107, 255})
+                    for (int nkb : {2048, 3125, 16400, 32767}) check(G, cap, S, nkb);
+        }
     std::printf("items %lld\n", items);
     return bad ? 1 : 0;
 }

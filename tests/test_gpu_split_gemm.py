@@ -9,19 +9,22 @@ from conftest import assert_bits_equal, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["f64", "int8", "int8_rocblas", "int8_tile256", "int8_hipblaslt", "bf16", "int8_residue", "f64_residue"]
+METHODS = ["f64", "int8", "int8_rocblas", "int8_tile256", "int8_hipblaslt", "bf16", "bf16_diag", "int8_residue",
+           "f64_residue"]
 
 
 def gemm(mdg, L, m, n, k, A, B, method, **kw):
-    """int8 runs the default engine (the hand-written band kernel), int8_mfma
-    forces it, int8_tile256 forces the hand-written single-diagonal 256 x 256
-    kernel, int8_rocblas / int8_hipblaslt the vendor libraries (comparison
+    """int8 and bf16 run the default engine (the hand-written band kernels),
+    int8_mfma forces it, int8_tile256 and bf16_diag force the hand-written
+    single-diagonal kernels (256 x 256 INT8, INT64-plane BF16),
+    int8_rocblas / int8_hipblaslt the vendor libraries (comparison
     engines); all give the same exact diagonal sums."""
-    engine = {"int8_rocblas": 1, "int8_mfma": 2, "int8_hipblaslt": 3, "int8_tile256": 4}.get(method)
-    if engine:
-        mdg.set_int8_backend(engine)
+    forced = {"int8_rocblas": (1, "int8"), "int8_mfma": (2, "int8"), "int8_hipblaslt": (3, "int8"),
+              "int8_tile256": (4, "int8"), "bf16_diag": (4, "bf16")}.get(method)
+    if forced:
+        mdg.set_int8_backend(forced[0])
         try:
-            return mdg.gemm(L, m, n, k, A, B, method="int8", **kw)
+            return mdg.gemm(L, m, n, k, A, B, method=forced[1], **kw)
         finally:
             mdg.set_int8_backend(0)
     return mdg.gemm(L, m, n, k, A, B, method=method, **kw)
@@ -316,33 +319,64 @@ def test_band_kernel_largest_partial_sums(mdg, orc, torch_cuda, method):
     assert err <= _bound(L, k), (err, info)
 
 
-def test_int8_deep_k_falls_back_to_tile256(mdg, orc, torch_cuda):
+def test_int8_deep_k_band_chunks(mdg, orc, torch_cuda):
     # one slice pair deeper than an INT32 plane allows (kp > 131071): the
-    # band kernel cannot cut its t-range finer, the 256 x 256 kernel runs
+    # band kernels cut the depth into K-chunks of whole k-blocks, the 256 x
+    # 256 kernel cuts it at other depths (130944 products), and every
+    # chunking must give the same exact sums
     L, m, n, k = 2, 8, 12, 131100
     A = orc.generate(orc.GEN_QD, L, m, k, 41, -2, 2)
     B = orc.generate(orc.GEN_QD, L, k, n, 42, -2, 2)
-    C, info = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), "int8")
-    assert info["backend"] == 4, info
+    dA, dB = to_dev(A), to_dev(B)
+    C, info = gemm(mdg, L, m, n, k, dA, dB, "int8")
+    assert info["backend"] == 2, info
     err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=96)
     assert err <= _bound(L, k), (err, info)
-    # BF16 at this depth: INT64 diagonal planes instead of INT32 chunks
-    C2, info2 = gemm(mdg, L, m, n, k, to_dev(A), to_dev(B), "bf16")
+    C4, info4 = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256")
+    assert info4["backend"] == 4, info4
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C4).view(np.uint64)), info4
+    C2, info2 = gemm(mdg, L, m, n, k, dA, dB, "bf16")
+    assert info2["backend"] == 2, info2
     assert np.array_equal(to_host(C).view(np.uint64), to_host(C2).view(np.uint64)), info2
 
 
-def test_bf16_many_band_items_falls_back(mdg, orc, torch_cuda):
-    # 107 slices at depth 34197 (3 slices per K-chunk): more BF16 band items
-    # than one launch's table holds; the single-diagonal BF16 kernel runs and
-    # gives the INT8 path's bits (this shape used to be refused)
+def test_deep_k_band_chunks_quad_double(mdg, orc, torch_cuda):
+    # a quad-double product three INT32 planes deep (k = 400000, 3125
+    # k-blocks of 128): INT8 and BF16 band kernels against the 256 x 256
+    # kernel, bit for bit, and against the exact product
+    L, m, n, k = 4, 20, 16, 400000
+    A = orc.generate(orc.GEN_QD, L, m, k, 51, -3, 3)
+    B = orc.generate(orc.GEN_QD, L, k, n, 52, -3, 3)
+    dA, dB = to_dev(A), to_dev(B)
+    C, info = gemm(mdg, L, m, n, k, dA, dB, "int8")
+    assert info["backend"] == 2, info
+    C4, _ = gemm(mdg, L, m, n, k, dA, dB, "int8_tile256")
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C4).view(np.uint64)), info
+    C2, info2 = gemm(mdg, L, m, n, k, dA, dB, "bf16")
+    assert info2["backend"] == 2, info2
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C2).view(np.uint64)), info2
+    C3, info3 = gemm(mdg, L, m, n, k, dA, dB, "bf16_diag")
+    assert info3["backend"] == 4, info3
+    assert np.array_equal(to_host(C).view(np.uint64), to_host(C3).view(np.uint64)), info3
+    err, _ = orc.normwise_error(L, m, n, k, A, B, to_host(C), nsamp=24)
+    assert err <= _bound(L, k), (err, info)
+
+
+def test_bf16_many_band_items(mdg, orc, torch_cuda):
+    # 107 slices at depth 34197: more BF16 band items than one launch's
+    # table holds, so the band kernel runs in several launches; it, the
+    # single-diagonal BF16 kernel and the INT8 path give the same bits
     L, m, n, k = 16, 24, 20, 34197
     A = orc.generate(orc.GEN_QD, L, m, k, 100, -46, 46)
     B = orc.generate(orc.GEN_QD, L, k, n, 200, -46, 46)
     dA, dB = to_dev(A), to_dev(B)
     C8, i8 = gemm(mdg, L, m, n, k, dA, dB, "int8", target_bits=824)
     C5, i5 = gemm(mdg, L, m, n, k, dA, dB, "bf16", target_bits=824)
+    C6, i6 = gemm(mdg, L, m, n, k, dA, dB, "bf16_diag", target_bits=824)
     assert i5["slices"] == i8["slices"] > 100, (i5, i8)
+    assert i5["backend"] == 2 and i6["backend"] == 4, (i5, i6)
     assert np.array_equal(to_host(C8).view(np.uint64), to_host(C5).view(np.uint64))
+    assert np.array_equal(to_host(C8).view(np.uint64), to_host(C6).view(np.uint64))
 
 
 def test_f64_residue_deep_flushes(mdg, orc, torch_cuda):

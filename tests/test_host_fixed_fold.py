@@ -108,7 +108,8 @@ def test_band_fold_equals_diagonal_fold(exe):
 
 def test_band_items_cover_every_plane_once(exe):
     # the band kernels' work items for slice counts 1..255, bands of 3 and 4,
-    # depths up to 2047 k-blocks and four chunk capacities: every plane the
+    # depths up to 2047 k-blocks (to 32767 for a few slice counts) and four
+    # chunk capacities: every plane the
     # epilogue writes lies inside the plane array and is written once, each
     # band's k-blocks are covered once, and each chunk's INT32 sums are exact
     r = subprocess.run([exe, "items"], capture_output=True, text=True, timeout=600)
