@@ -23,13 +23,15 @@
 // blocks of the reference's Toeplitz operand, never materialised here.
 //
 // Three GEMM back ends share the slicing:
-//   INT8  w = 8   INT8 x INT8 -> INT32     exact while the depth K of one
-//                 (rocBLAS gemm_ex, or     plane has K * 2^14 < 2^31; deeper
-//                 v_mfma_i32_16x16x64_i8)  diagonals are cut into K-chunk
-//                                          planes added in INT64 afterwards;
-//                                          the engine is autotuned per shape
-//   BF16  w = 8   v_mfma_f32_16x16x32_bf16 exact FP32 sums (< 2^24), widened
-//                                          to INT64 every 1024 k
+//   INT8  w = 8   v_mfma_i32_32x32x32_i8   exact INT32 sums while the depth K
+//                 (the band kernel,        of one plane has K * 2^14 < 2^31;
+//                 i8band.hpp, by default;  deeper bands are cut into K-chunks,
+//                 the 256 x 256 diagonal   each writing one INT64 band plane,
+//                 kernel or rocBLAS /      added in the reconstruction
+//                 hipBLASLt on request)
+//   BF16  w = 8   v_mfma_f32_32x32x16_bf16 exact FP32 sums (< 2^24), moved
+//                 (band kernel; 16x16x32   into INT32 within each K-chunk
+//                 in the diagonal kernel)
 //   FP64  w <= 25 v_mfma_f64_16x16x4_f64   exact FP64 sums (< 2^53), widened
 //                                          to INT64 every 2^(55-2w) k
 #include <climits>
