@@ -597,24 +597,29 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
                             mod_pos(fma((double)h, 65536.0, fma((double)x, 256.0, (double)l)), t.gP[g], t.ginv[g]));
             }
         }
-        // Z = sum R_g c_g - q M in two's-complement 24-bit words
+        // Z = sum R_g c_g - q M in two's-complement 24-bit words.  The carry
+        // chain stays in FP64: every v below is an integer with |v| <
+        // 2^52.2 + 2^29 < 2^53 (the column sums' bound plus a carry), so
+        // floor(v 2^-24) and v - 2^24 floor(v 2^-24) are exact -- the words
+        // and carries of an INT64 chain, without its conversions
         const double q = rint(tq);
-        long long z[NW];
-        long long carry = 0;
+        double z[NW];
+        double carry = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            const long long v = (long long)fma(-q, t.Mw[w], acc[w]) + carry;
-            z[w] = v & 0xFFFFFF;
-            carry = v >> 24;
+            const double v = fma(-q, t.Mw[w], acc[w]) + carry;
+            carry = floor(v * 0x1p-24);
+            z[w] = fma(-carry, 16777216.0, v);
         }
-        const bool neg = carry < 0;
+        const bool neg = carry < 0.0;
         if (neg) {
-            long long cy = 1;
+            // |Z| = the two's complement: (2^24 - 1 - z) words plus one
+            double cy = 1.0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
-                const long long v = (0xFFFFFF - z[w]) + cy;
-                z[w] = v & 0xFFFFFF;
-                cy = v >> 24;
+                const double v = (16777215.0 - z[w]) + cy;
+                cy = (v >= 16777216.0) ? 1.0 : 0.0;
+                z[w] = v - 16777216.0 * cy;
             }
         }
         // 48-bit pieces, the leading one first: K pieces carry at least
@@ -625,7 +630,7 @@ __device__ __forceinline__ void residue_reconstruct_at(long long idx, int m, int
         int tp = -1;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            d[p] = fma((double)z[2 * p + 1], 16777216.0, (double)z[2 * p]);
+            d[p] = fma(z[2 * p + 1], 16777216.0, z[2 * p]);
             tp = (d[p] != 0.0) ? p : tp;
         }
         if (tp >= 0) {
